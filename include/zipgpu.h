@@ -1,0 +1,141 @@
+/*
+ * zipgpu.h -- C ABI of the MI355X (gfx950) Zip/Zip+ commitment library.
+ *
+ * This is the drop-in boundary for the reference's commitment hot path
+ * (NethermindEth/zinc-plus, crate zip-plus).  Every entry point names the
+ * reference interface it replaces.  All signatures use plain pointers and
+ * sizes; "d_" pointers are device (HBM) pointers, "h_" pointers are host
+ * pointers.  A stream argument of NULL means the default HIP stream.
+ *
+ * Element layouts are the reference's in-memory layouts with the `simd`
+ * feature (poly/src/univariate.rs:41-50):
+ *   BinaryPoly<D>        u64, bit c = coefficient c   (binary_u64.rs:26-28)
+ *   i32 / i128           little-endian two's complement
+ *   DensePolynomial<T,D> [T; D], coefficient 0 first  (dense.rs:30-34)
+ *   Int<L>               L little-endian u64 limbs, two's complement
+ * so codeword buffers are byte-identical to the reference's DenseRowMatrix
+ * (row-major num_rows x codeword_len) and to the Merkle leaf bytes
+ * (transcript/src/traits.rs:432-517).
+ *
+ * Functions return a zp_status.  Where the reference returns
+ * Err(ZipError::InvalidPcsParam) the ABI returns ZP_ERR_INVALID_PARAM; where
+ * it panics on an assert it returns ZP_ERR_SHAPE.  zp_last_error() gives the
+ * message of the calling thread's last failure.
+ *
+ * Thread safety: codes are immutable after creation and may be shared by any
+ * number of threads; hints are owned by one caller at a time.
+ */
+#ifndef ZIPGPU_H
+#define ZIPGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  ZP_OK = 0,
+  ZP_ERR_INVALID_PARAM = 1, /* ZipError::InvalidPcsParam (zip-plus/src/lib.rs:13-29) */
+  ZP_ERR_OVERFLOW = 2,      /* the CHECK "should not overflow" panics */
+  ZP_ERR_SHAPE = 3,         /* assert! panics on shapes */
+  ZP_ERR_DEVICE = 4,        /* HIP runtime failure */
+  ZP_ERR_OOM = 5            /* device allocation failed / admission refused */
+} zp_status;
+
+typedef enum { ZP_CODE_IPRS = 0, ZP_CODE_RAA = 1, ZP_CODE_RAA_SIGNFLIP = 2 } zp_code_kind;
+
+typedef enum {
+  ZP_EVAL_I32 = 0,    /* Eval = i32, Cw = i128 (zip-plus/benches/zip_benches.rs:25-40) */
+  ZP_EVAL_BINPOLY = 1 /* Eval = BinaryPoly<D>, Cw = [i64;D] (IPRS) or [i32;D] (RAA),
+                         zip-plus/benches/zip_plus_benches.rs:37-78 */
+} zp_eval_type;
+
+typedef struct zp_code zp_code;
+typedef struct zp_hint zp_hint;
+typedef void* zp_stream; /* hipStream_t */
+
+/* Library set-up on a device (selects it for the calling thread). */
+int zp_init(int device);
+const char* zp_status_string(int status);
+const char* zp_last_error(void);
+
+/* ---- LinearCode (zip-plus/src/code.rs:10-72) -------------------------- */
+
+/* IprsCode::new(row_len, depth) (zip-plus/src/code/iprs.rs:30-37; errors as
+ * Radix8PnttParams::new, code/iprs/pntt/radix8/params.rs:59-103), or
+ * IprsCode::new_with_optimal_depth (iprs.rs:43-51) when depth < 0.
+ * RaaCode::new(row_len) (zip-plus/src/code/raa.rs:38-90, perms from seeds 1
+ * and 2) and RaaSignFlippingCode::new (code/raa_sign_flip.rs) ignore depth.
+ * rep is the const REP (REPETITION_FACTOR). */
+int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_code** out);
+void zp_code_destroy(zp_code* code);
+/* LinearCode::row_len / codeword_len; depth is Radix8PnttParams::depth. */
+uint64_t zp_code_row_len(const zp_code* code);
+uint64_t zp_code_codeword_len(const zp_code* code);
+uint32_t zp_code_depth(const zp_code* code);
+int zp_code_kind_of(const zp_code* code);
+
+/* Bytes of one codeword element for an eval type (16 for i128, 8*D or 4*D
+ * for binary polys under IPRS / RAA). */
+uint64_t zp_cw_elem_bytes(const zp_code* code, int eval_type, uint32_t degree_plus_one);
+
+/* ZipPlus::encode_rows (zip-plus/src/pcs/phase_commit.rs:159-176), i.e.
+ * LinearCode::encode (iprs.rs:120-130, raa.rs:147-149) over num_rows
+ * consecutive rows of row_len evals.  d_out receives num_rows x
+ * codeword_len elements, row-major.  Asynchronous on `stream`. */
+int zp_encode_rows(const zp_code* code, int eval_type, uint32_t degree_plus_one,
+                   const void* d_evals, uint64_t num_rows, void* d_out, zp_stream stream);
+
+/* ---- Merkle tree (zip-plus/src/merkle.rs) ----------------------------- */
+
+/* MerkleTree::new(rows).root() (merkle.rs:74-102) over a device row matrix
+ * of num_rows x width elements of elem_bytes each. */
+int zp_merkle_root(const void* d_rows, uint64_t num_rows, uint64_t width, uint32_t elem_bytes,
+                   uint8_t h_root[32], zp_stream stream);
+
+/* ---- Commit (zip-plus/src/pcs/phase_commit.rs:61-101) ----------------- */
+
+/* ZipPlus::commit(pp, polys) for `batch` polys of num_rows*row_len evals,
+ * stored contiguously (poly-major) at d_evals.  The hint (ZipPlusHint,
+ * pcs/structs.rs:114-129) stays on the device: codewords and every Merkle
+ * layer.  h_root receives ZipPlusCommitment::root. */
+int zp_commit(const zp_code* code, int eval_type, uint32_t degree_plus_one, const void* d_evals,
+              uint64_t num_rows, uint32_t batch, zp_hint** hint_out, uint8_t h_root[32],
+              zp_stream stream);
+
+/* Hint accessors (the reference keeps these in host memory). */
+int zp_hint_info(const zp_hint* hint, uint64_t* cw_bytes_per_poly, uint32_t* batch,
+                 uint64_t* num_leaves, uint32_t* height);
+const void* zp_hint_codewords(const zp_hint* hint); /* device pointer */
+int zp_hint_copy_codewords(const zp_hint* hint, uint32_t poly, void* h_out, zp_stream stream);
+/* Layer `level` of the tree (0 = leaves ... height-1 = root), 32 B each. */
+int zp_hint_copy_layer(const zp_hint* hint, uint32_t level, uint8_t* h_out, zp_stream stream);
+/* MerkleTree::prove(leaf) (merkle.rs:105-120, 224-257): siblings bottom-up. */
+int zp_merkle_prove(const zp_hint* hint, uint64_t leaf, uint8_t* h_siblings, uint32_t* h_len,
+                    zp_stream stream);
+/* Column `col` over all rows of all polys (the bytes a column opening sends,
+ * zip-plus/src/pcs/phase_prove.rs:247-304): batch*num_rows elements. */
+int zp_hint_open_column(const zp_hint* hint, uint64_t col, void* h_out, zp_stream stream);
+void zp_hint_free(zp_hint* hint);
+
+/* ---- Prover combined row (zip-plus/src/pcs/phase_prove.rs:153-244) ----- */
+
+/* For `batch` binary polys (D = degree_plus_one) of num_rows x row_len evals:
+ *   w'_b[k]  = sum of alpha_{b,c} over the set bits c of eval k
+ *              (EvalDotChal, poly/src/univariate/binary_u64.rs:393-432)
+ *   comb[col] = sum_b sum_j s_j * w'_b[j*row_len + col]
+ * in CombR = Int<comb_limbs> (two's complement mod 2^(64*comb_limbs)).
+ * d_alphas: batch x D i128; d_coeffs: num_rows i128 (the reference uses [1]
+ * when num_rows == 1); d_out: row_len x comb_limbs u64. */
+int zp_combined_row(uint32_t degree_plus_one, uint32_t comb_limbs, uint64_t num_rows,
+                    uint64_t row_len, uint32_t batch, const uint64_t* d_evals,
+                    const int64_t* d_alphas, const int64_t* d_coeffs, uint64_t* d_out,
+                    zp_stream stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ZIPGPU_H */

@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every entry point of include/zipgpu.h
+(CPU only: no compute calls)."""
+import ctypes
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "zipgpu.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(zp_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_header_declares_entry_points():
+    names = declared_functions()
+    for must in ["zp_code_create", "zp_encode_rows", "zp_commit", "zp_merkle_root",
+                 "zp_merkle_prove", "zp_combined_row", "zp_hint_free"]:
+        assert must in names
+
+
+def test_library_exports_every_declared_symbol():
+    from zipgpu import _lib
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_binding_covers_header():
+    from zipgpu import _lib
+    assert sorted(_lib.SIGNATURES) == declared_functions()
+
+
+def test_status_strings_without_device():
+    from zipgpu import _lib
+    lib = _lib.load()
+    assert lib.zp_status_string(0) == b"ok"
+    assert lib.zp_status_string(1) == b"invalid parameter"
+
+
+def test_setup_validation():
+    # pcs/structs.rs:74-88 asserts; checked on the host before any device work.
+    import zipgpu
+
+    class Code:
+        def row_len(self):
+            return 256
+
+    assert zipgpu.ZipPlus.setup(1 << 16, Code()).num_rows == 256
+    with pytest.raises(zipgpu.ShapeError):
+        zipgpu.ZipPlus.setup(3 << 10, Code())
+    with pytest.raises(zipgpu.ShapeError):
+        zipgpu.ZipPlus.setup(128, Code())

@@ -1,0 +1,258 @@
+"""Bit-exact parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Inputs are seeded (zipgpu.synth); every codeword word, Merkle root and
+combined-row limb must equal the oracle's.
+"""
+import numpy as np
+import pytest
+import torch
+
+from zipgpu import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def to_dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+# ---------------------------------------------------------------------------
+# IPRS encode
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("row_len,depth,D,rows", [
+    (256, 1, 32, 3), (512, -1, 64, 2), (1 << 12, 2, 32, 2), (1 << 14, 2, 32, 2),
+    (1 << 14, 2, 64, 1), (32, 1, 3, 4), (2048, 3, 32, 1), (64, 0, 16, 2), (8, 1, 1, 8),
+    (1 << 13, -1, 7, 1),
+])
+def test_iprs_binary_encode(zg, dev, oracle, row_len, depth, D, rows):
+    say("iprs_binary", row_len, depth, D, rows)
+    x = synth.binary_polys(100 + row_len + D, rows * row_len, D)
+    code = zg.IprsCode(row_len, depth)
+    types = zg.BinPolyTypes(D)
+    got = code.encode_rows(to_dev(x, dev), rows, types).cpu().numpy()
+    want = oracle.iprs_encode_binary(x.view(np.uint64), rows, row_len, depth, D)
+    assert np.array_equal(got, want)
+
+
+def test_iprs_binary_ignores_bits_above_degree(zg, dev, oracle):
+    # BinaryPoly<D> only has D coefficients; higher bits of the u64 lane are
+    # not read (binary_u64.rs:445-483, dense.rs:508-525).
+    row_len, D = 512, 20
+    x = synth.splitmix64(7, row_len).view(np.int64)
+    code = zg.IprsCode.new_with_optimal_depth(row_len)
+    got = code.encode(to_dev(x, dev), zg.BinPolyTypes(D)).cpu().numpy()
+    assert np.array_equal(got, oracle.iprs_encode_binary(x.view(np.uint64), 1, row_len, -1, D)[0])
+
+
+@pytest.mark.parametrize("row_len,depth,rows", [
+    (256, 1, 2), (1 << 14, 2, 4), (2048, 3, 1), (128, 0, 1), (1 << 12, -1, 3)])
+def test_iprs_i32_encode(zg, dev, oracle, row_len, depth, rows):
+    say("iprs_i32", row_len, depth, rows)
+    x = synth.i32_evals(200 + row_len, rows * row_len)
+    code = zg.IprsCode(row_len, depth)
+    got = code.encode_rows(to_dev(x, dev), rows, zg.IntTypes()).cpu().numpy()
+    want = oracle.iprs_encode_i32(x, rows, row_len, depth)
+    assert np.array_equal(got, want)
+
+
+def test_iprs_extreme_inputs(zg, dev, oracle):
+    # All-ones binary rows and i32::MIN/MAX rows give the largest magnitudes.
+    row_len = 1 << 14
+    ones = np.full(row_len, -1, dtype=np.int64)
+    code = zg.IprsCode.new_with_optimal_depth(row_len)
+    got = code.encode(to_dev(ones, dev), zg.BinPolyTypes(64)).cpu().numpy()
+    assert np.array_equal(got, oracle.iprs_encode_binary(ones.view(np.uint64), 1, row_len, -1,
+                                                         64)[0])
+    for v in (-2**31, 2**31 - 1):
+        xi = np.full(row_len, v, dtype=np.int32)
+        got = code.encode(to_dev(xi, dev), zg.IntTypes()).cpu().numpy()
+        assert np.array_equal(got, oracle.iprs_encode_i32(xi, 1, row_len)[0])
+
+
+@pytest.mark.parametrize("row_len,depth", [(1 << 15, 2), (12, 1), (1, 1), (24 * 8, 1)])
+def test_iprs_invalid_params(zg, row_len, depth):
+    # params.rs:59-80, iprs.rs:225-235
+    with pytest.raises(zg.InvalidPcsParam):
+        zg.IprsCode(row_len, depth)
+
+
+def test_encode_wrong_length(zg, dev):
+    code = zg.IprsCode(256, 1)
+    with pytest.raises(zg.ShapeError):
+        code.encode(torch.zeros(255, dtype=torch.int64, device=dev), zg.BinPolyTypes(32))
+
+
+# ---------------------------------------------------------------------------
+# RAA encode
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("row_len,D,rows,flip", [
+    (4, 32, 2, False), (64, 32, 3, False), (2048, 64, 2, False), (2048, 32, 2, True),
+    (1 << 13, 3, 1, False), (1, 8, 4, False), (256, 64, 5, True)])
+def test_raa_binary_encode(zg, dev, oracle, row_len, D, rows, flip):
+    say("raa", row_len, D, rows, flip)
+    x = synth.binary_polys(300 + row_len, rows * row_len, D)
+    code = zg.RaaSignFlippingCode(row_len) if flip else zg.RaaCode(row_len)
+    got = code.encode_rows(to_dev(x, dev), rows, zg.BinPolyTypes(D)).cpu().numpy()
+    want = oracle.raa_encode_binary(x.view(np.uint64), rows, row_len, D, sign_flip=flip)
+    assert np.array_equal(got, want)
+
+
+def test_raa_width_assert(zg, dev):
+    # raa.rs:52-69: 1 + 2*14 + 2*2 = 33 bits do not fit i32 codewords.
+    code = zg.RaaCode(1 << 14)
+    with pytest.raises(zg.ShapeError):
+        code.encode(torch.zeros(1 << 14, dtype=torch.int64, device=dev), zg.BinPolyTypes(32))
+
+
+def test_raa_known_answer_pattern(zg, dev, oracle):
+    # The reference's KAT row [1,2,3,4] (raa.rs:396-413), bit plane by plane:
+    # sum_c 2^c * plane_c equals the integer encoding of the row.
+    row = np.array([1, 2, 3, 4], dtype=np.int64)
+    code = zg.RaaCode(4)
+    got = code.encode(to_dev(row, dev), zg.BinPolyTypes(3)).cpu().numpy().astype(np.int64)
+    combined = (got * (1 << np.arange(3))).sum(axis=1)
+    assert list(combined) == [0x1E, 0x36, 0x39, 0x5A, 0x70, 0x7E, 0xA5, 0xC1, 0xCB, 0xDC, 0xF9,
+                              0x11E, 0x124, 0x14C, 0x14D, 0x160]
+
+
+# ---------------------------------------------------------------------------
+# Merkle
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("rows,width,elem", [
+    (1, 1, 4), (4, 16, 16), (64, 1024, 256), (5, 64, 256), (3, 8, 24), (2048, 64, 256),
+    (1, 4, 8), (17, 32, 12)])
+def test_merkle_root(zg, dev, oracle, rows, width, elem):
+    say("merkle", rows, width, elem)
+    m = synth.splitmix64(rows * 1000 + width, rows * width * elem // 8 + 1).view(np.uint8)
+    m = m[:rows * width * elem].reshape(rows, width, elem)
+    got = zg.merkle_root(to_dev(m, dev), elem)
+    assert got == oracle.merkle_root(m, rows, width, elem)
+
+
+# ---------------------------------------------------------------------------
+# Commit (encode + Merkle) and the hint
+# ---------------------------------------------------------------------------
+
+def test_commit_plain_zip_cfg1(zg, dev, oracle):
+    # n = 2^16 i32 evals, IPRS rate 1/4, row_len 2^14 x 4 rows (depth 2).
+    n, row_len = 1 << 16, 1 << 14
+    x = synth.i32_evals(1, n)
+    code = zg.IprsCode.new_with_optimal_depth(row_len)
+    zp = zg.ZipPlus(zg.IntTypes())
+    pp = zp.setup(n, code)
+    hint, comm = zp.commit(pp, [to_dev(x, dev)])
+    ref = oracle.commit("iprs", x, pp.num_rows, row_len, eval_type="i32", want_cw=True)
+    assert comm.root == ref["root"] and comm.batch_size == 1
+    assert hint.codewords(0).tobytes() == ref["cw"].tobytes()
+
+
+@pytest.mark.parametrize("kind,n,row_len,D,batch", [
+    ("iprs", 1 << 16, 1 << 14, 32, 1), ("iprs", 1 << 12, 256, 64, 2), ("raa", 1 << 14, 128, 64, 1),
+    ("raa", 1 << 12, 64, 32, 5), ("raa_signflip", 1 << 12, 64, 32, 1),
+    ("iprs", 1 << 10, 256, 3, 3)])
+def test_commit_binary(zg, dev, oracle, kind, n, row_len, D, batch):
+    say("commit", kind, n, row_len, D, batch)
+    x = synth.binary_polys(400 + n + D, batch * n, D)
+    code = {"iprs": lambda: zg.IprsCode.new_with_optimal_depth(row_len),
+            "raa": lambda: zg.RaaCode(row_len),
+            "raa_signflip": lambda: zg.RaaSignFlippingCode(row_len)}[kind]()
+    zp = zg.ZipPlus(zg.BinPolyTypes(D))
+    pp = zp.setup(n, code)
+    polys = [to_dev(x[b * n:(b + 1) * n], dev) for b in range(batch)]
+    hint, comm = zp.commit(pp, polys)
+    ref = oracle.commit(kind, x.view(np.uint64), pp.num_rows, row_len, batch=batch, D=D,
+                        want_cw=True, want_layers=True)
+    assert comm.root == ref["root"]
+    per = ref["cw"].reshape(batch, -1)
+    for b in range(batch):
+        assert hint.codewords(b).tobytes() == per[b].tobytes()
+    assert np.array_equal(hint.layer(0), ref["layers"][:code.codeword_len()])
+
+
+def test_commit_zip_plus_cfg2_full(zg, dev, oracle):
+    # The headline workload at full size: 2^20 BinaryPoly<32>, IPRS row 2^14
+    # x 64 rows (depth 2); every codeword byte and the root.
+    n, row_len, D = 1 << 20, 1 << 14, 32
+    x = synth.binary_polys(2, n, D)
+    code = zg.IprsCode.new_with_optimal_depth(row_len)
+    zp = zg.ZipPlus(zg.BinPolyTypes(D))
+    pp = zp.setup(n, code)
+    hint, comm = zp.commit(pp, [to_dev(x, dev)])
+    ref = oracle.commit("iprs", x.view(np.uint64), pp.num_rows, row_len, D=D, want_cw=True)
+    assert comm.root == ref["root"]
+    assert hint.codewords(0).tobytes() == ref["cw"].tobytes()
+
+
+def test_hint_openings_verify(zg, dev, oracle):
+    n, row_len, D = 1 << 12, 256, 32
+    x = synth.binary_polys(77, 2 * n, D)
+    code = zg.IprsCode.new_with_optimal_depth(row_len)
+    zp = zg.ZipPlus(zg.BinPolyTypes(D))
+    pp = zp.setup(n, code)
+    hint, comm = zp.commit(pp, [to_dev(x[:n], dev), to_dev(x[n:], dev)])
+    N = code.codeword_len()
+    for col in [0, 1, 17, N - 1]:
+        proof = hint.prove(col)
+        assert len(proof.siblings) == N.bit_length() - 1
+        column = hint.open_column(col)
+        assert oracle.merkle_verify(comm.root, column, col, N, proof.siblings)
+        bad = bytearray(column)
+        bad[-1] ^= 0x80
+        assert not oracle.merkle_verify(comm.root, bytes(bad), col, N, proof.siblings)
+    with pytest.raises(zg.InvalidPcsParam):
+        hint.prove(N)
+
+
+def test_commit_shape_errors(zg, dev):
+    code = zg.IprsCode(256, 1)
+    zp = zg.ZipPlus(zg.BinPolyTypes(32))
+    pp = zp.setup(1 << 10, code)
+    with pytest.raises(zg.ShapeError):
+        zp.commit(pp, [])
+    with pytest.raises(zg.ShapeError):
+        zp.commit(pp, [torch.zeros(512, dtype=torch.int64, device=dev)])
+
+
+def test_commit_linearity_full_size(zg, dev):
+    # Size-independent property at the headline size: for bitwise-disjoint
+    # inputs a, b the IPRS codewords add, enc(a | b) = enc(a) + enc(b).
+    n, row_len, D = 1 << 20, 1 << 14, 32
+    x = synth.binary_polys(8, n, D)
+    m = synth.binary_polys(9, n, D)
+    a, b = x & m, x & ~m & ((1 << D) - 1)
+    code = zg.IprsCode.new_with_optimal_depth(row_len)
+    types = zg.BinPolyTypes(D)
+    rows = n // row_len
+    ea = code.encode_rows(to_dev(a, dev), rows, types)
+    eb = code.encode_rows(to_dev(b, dev), rows, types)
+    ex = code.encode_rows(to_dev(a | b, dev), rows, types)
+    assert torch.equal(ea + eb, ex)
+
+
+# ---------------------------------------------------------------------------
+# Combined row
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("rows,row_len,D,batch,L", [
+    (256, 256, 64, 1, 5), (64, 128, 32, 2, 5), (1, 64, 64, 1, 5), (16, 32, 3, 3, 4),
+    (2048, 64, 64, 1, 5)])
+def test_combined_row(zg, dev, oracle, rows, row_len, D, batch, L):
+    say("combined_row", rows, row_len, D, batch, L)
+    n = rows * row_len
+    x = synth.binary_polys(500 + rows, batch * n, D)
+    alphas = synth.i128_words(501, batch * D)
+    coeffs = synth.i128_words(502, rows) if rows > 1 else np.array([[1, 0]], dtype=np.int64)
+    code = zg.RaaCode(row_len)
+    zp = zg.ZipPlus(zg.BinPolyTypes(D, comb_limbs=L))
+    pp = zp.setup(n, code)
+    polys = [to_dev(x[b * n:(b + 1) * n], dev) for b in range(batch)]
+    got = zp.combined_row(pp, polys, to_dev(alphas, dev), to_dev(coeffs, dev)).cpu().numpy()
+    want = oracle.combined_row(x.view(np.uint64), alphas, coeffs, rows, row_len, batch, D, L)
+    assert np.array_equal(got.view(np.uint64), want)

@@ -1,0 +1,330 @@
+// api.hip -- the C ABI declared in include/zipgpu.h.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "common.h"
+
+namespace zpi {
+
+static thread_local std::string g_last_error;
+
+int fail(int status, const std::string& msg) {
+  g_last_error = msg;
+  return status;
+}
+
+}  // namespace zpi
+
+using namespace zpi;
+
+struct zp_hint {
+  int device = 0;
+  uint8_t* d_cw = nullptr;      // batch x num_rows x N elements, poly-major
+  uint8_t* d_layers = nullptr;  // (2N-1) x 32 bytes, leaves first, root last
+  uint64_t N = 0, num_rows = 0, elem = 0, per_poly = 0;
+  uint32_t batch = 0, height = 0;
+};
+
+namespace {
+
+bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+template <class T>
+int upload(const std::vector<T>& host, T** dev) {
+  ZP_HIP(hipMalloc((void**)dev, host.size() * sizeof(T)));
+  ZP_HIP(hipMemcpy(*dev, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+  return ZP_OK;
+}
+
+void free_code(zp_code* c) {
+  if (!c) return;
+  (void)hipFree(c->d_ftab);
+  for (auto* p : c->d_tw) (void)hipFree(p);
+  (void)hipFree(c->d_perm1);
+  (void)hipFree(c->d_perm2);
+  delete c;
+}
+
+uint64_t elem_bytes(const zp_code* c, int eval_type, uint32_t D) {
+  if (c->kind == ZP_CODE_IPRS) {
+    if (eval_type == ZP_EVAL_I32) return 16;
+    if (eval_type == ZP_EVAL_BINPOLY) return 8ull * D;
+  } else if (eval_type == ZP_EVAL_BINPOLY) {
+    return 4ull * D;
+  }
+  return 0;
+}
+
+int check_types(const zp_code* c, int eval_type, uint32_t D) {
+  if (eval_type == ZP_EVAL_BINPOLY && (D == 0 || D > 64))
+    return fail(ZP_ERR_INVALID_PARAM, "BinaryPoly degree bound must be within 1..64");
+  if (elem_bytes(c, eval_type, D) == 0)
+    return fail(ZP_ERR_INVALID_PARAM, "unsupported eval type for this code");
+  if (c->kind != ZP_CODE_IPRS) {
+    // raa.rs:52-69: 1 + 2*log2(row_len) + 2*log2(REP) bits must fit i32.
+    const uint64_t w = 1 + 2 * (uint64_t)__builtin_ctzll(c->row_len) +
+                       2 * (uint64_t)__builtin_ctzll(c->rep);
+    if (w > 32)
+      return fail(ZP_ERR_SHAPE, "Cannot fit " + std::to_string(w) +
+                                    "-bit wide codeword entries in 32 bits entries");
+  }
+  return ZP_OK;
+}
+
+int encode(const zp_code* c, int eval_type, uint32_t D, const void* evals, uint64_t num_rows,
+           void* out, hipStream_t s) {
+  ZP_TRY(check_types(c, eval_type, D));
+  if (c->kind == ZP_CODE_IPRS) {
+    if (eval_type == ZP_EVAL_BINPOLY)
+      return launch_iprs_binary(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
+    return launch_iprs_i32(c, (const int32_t*)evals, num_rows, out, s);
+  }
+  return launch_raa_binary(c, (const uint64_t*)evals, num_rows, D, (int32_t*)out, s);
+}
+
+uint64_t eval_bytes(int eval_type) { return eval_type == ZP_EVAL_I32 ? 4 : 8; }
+
+}  // namespace
+
+extern "C" {
+
+int zp_init(int device) {
+  ZP_HIP(hipSetDevice(device));
+  return ZP_OK;
+}
+
+const char* zp_status_string(int status) {
+  switch (status) {
+    case ZP_OK: return "ok";
+    case ZP_ERR_INVALID_PARAM: return "invalid parameter";
+    case ZP_ERR_OVERFLOW: return "overflow";
+    case ZP_ERR_SHAPE: return "shape";
+    case ZP_ERR_DEVICE: return "device";
+    case ZP_ERR_OOM: return "out of memory";
+  }
+  return "unknown";
+}
+
+const char* zp_last_error(void) { return g_last_error.c_str(); }
+
+int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_code** out) {
+  if (!out) return fail(ZP_ERR_INVALID_PARAM, "null output");
+  *out = nullptr;
+  zp_code* c = new zp_code();
+  c->kind = kind;
+  c->rep = rep;
+  c->row_len = row_len;
+  c->N = row_len * rep;
+  if (hipGetDevice(&c->device) != hipSuccess) c->device = 0;
+  int rc = ZP_OK;
+  if (kind == ZP_CODE_IPRS) {
+    const uint64_t d = depth < 0 ? iprs_optimal_depth(row_len) : (uint64_t)depth;
+    uint32_t bl = 0, bd = 0;
+    rc = iprs_params(row_len, d, rep, &bl, &bd);
+    if (rc == ZP_OK && d > kMaxIprsDepth)
+      rc = fail(ZP_ERR_INVALID_PARAM, "IPRS depth > 3 is not supported on the device");
+    if (rc == ZP_OK) {
+      c->depth = (uint32_t)d;
+      c->bl = bl;
+      c->bd = bd;
+      c->bd_log2 = (uint32_t)__builtin_ctz(bd);
+      std::vector<int32_t> f(bd);
+      iprs_ftab(bd, f.data());
+      rc = upload(f, &c->d_ftab);
+      for (uint32_t k = 0; rc == ZP_OK && k < c->depth; k++) {
+        std::vector<int32_t> tw(((uint64_t)bd << (3 * k)) * 56);
+        iprs_twiddles(bd, c->N, c->depth, k, tw.data());
+        rc = upload(tw, &c->d_tw[k]);
+      }
+    }
+  } else if (kind == ZP_CODE_RAA || kind == ZP_CODE_RAA_SIGNFLIP) {
+    if (!pow2(rep)) rc = fail(ZP_ERR_SHAPE, "Repetition factor must be a power of two");
+    else if (!pow2(row_len)) rc = fail(ZP_ERR_SHAPE, "Row length must be a power of two");
+    else if (c->N > (1ull << 32)) rc = fail(ZP_ERR_INVALID_PARAM, "codeword too long");
+    if (rc == ZP_OK) {
+      std::vector<uint32_t> p1(c->N), p2(c->N);
+      raa_perm(c->N, 1, p1.data());
+      raa_perm(c->N, 2, p2.data());
+      rc = upload(p1, &c->d_perm1);
+      if (rc == ZP_OK) rc = upload(p2, &c->d_perm2);
+    }
+  } else {
+    rc = fail(ZP_ERR_INVALID_PARAM, "unknown code kind");
+  }
+  if (rc != ZP_OK) {
+    free_code(c);
+    return rc;
+  }
+  *out = c;
+  return ZP_OK;
+}
+
+void zp_code_destroy(zp_code* code) { free_code(code); }
+uint64_t zp_code_row_len(const zp_code* c) { return c ? c->row_len : 0; }
+uint64_t zp_code_codeword_len(const zp_code* c) { return c ? c->N : 0; }
+uint32_t zp_code_depth(const zp_code* c) { return c ? c->depth : 0; }
+int zp_code_kind_of(const zp_code* c) { return c ? c->kind : -1; }
+
+uint64_t zp_cw_elem_bytes(const zp_code* c, int eval_type, uint32_t D) {
+  return c ? elem_bytes(c, eval_type, D) : 0;
+}
+
+int zp_encode_rows(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
+                   uint64_t num_rows, void* d_out, zp_stream stream) {
+  if (!code) return fail(ZP_ERR_INVALID_PARAM, "null code");
+  if (num_rows && (!d_evals || !d_out)) return fail(ZP_ERR_INVALID_PARAM, "null buffer");
+  return encode(code, eval_type, D, d_evals, num_rows, d_out, (hipStream_t)stream);
+}
+
+int zp_merkle_root(const void* d_rows, uint64_t num_rows, uint64_t width, uint32_t elem,
+                   uint8_t h_root[32], zp_stream stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (num_rows == 0 || width == 0) return fail(ZP_ERR_SHAPE, "empty matrix");
+  if (!pow2(width)) return fail(ZP_ERR_SHAPE, "row width must be a power of two");
+  uint8_t* layers = nullptr;
+  ZP_HIP(hipMalloc((void**)&layers, (2 * width - 1) * 32));
+  int rc = launch_leaf_hash((const uint8_t*)d_rows, num_rows, width, elem, layers, s);
+  if (rc == ZP_OK) rc = launch_merkle_layers(layers, width, s);
+  if (rc == ZP_OK && hipMemcpyAsync(h_root, layers + (2 * width - 2) * 32, 32,
+                                    hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "root copy failed");
+  if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
+  (void)hipFree(layers);
+  return rc;
+}
+
+int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
+              uint64_t num_rows, uint32_t batch, zp_hint** hint_out, uint8_t h_root[32],
+              zp_stream stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!code || !hint_out) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  *hint_out = nullptr;
+  if (batch == 0) return fail(ZP_ERR_SHAPE, "Batch must contain at least one polynomial");
+  if (num_rows == 0) return fail(ZP_ERR_SHAPE, "num_rows must be positive");
+  ZP_TRY(check_types(code, eval_type, D));
+  const uint64_t N = code->N, elem = elem_bytes(code, eval_type, D);
+  if (!pow2(N)) return fail(ZP_ERR_SHAPE, "codeword length must be a power of two");
+  zp_hint* h = new zp_hint();
+  h->device = code->device;
+  h->N = N;
+  h->num_rows = num_rows;
+  h->elem = elem;
+  h->per_poly = num_rows * N * elem;
+  h->batch = batch;
+  h->height = (uint32_t)__builtin_ctzll(N) + 1;
+  int rc = ZP_OK;
+  if (hipMalloc((void**)&h->d_cw, h->per_poly * batch) != hipSuccess ||
+      hipMalloc((void**)&h->d_layers, (2 * N - 1) * 32) != hipSuccess) {
+    (void)hipGetLastError();
+    rc = fail(ZP_ERR_OOM, "device allocation for the commitment hint failed");
+  }
+  const uint64_t in_stride = num_rows * code->row_len * eval_bytes(eval_type);
+  for (uint32_t b = 0; rc == ZP_OK && b < batch; b++)
+    rc = encode(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride, num_rows,
+                h->d_cw + b * h->per_poly, s);
+  if (rc == ZP_OK) rc = launch_leaf_hash(h->d_cw, num_rows * batch, N, (uint32_t)elem,
+                                         h->d_layers, s);
+  if (rc == ZP_OK) rc = launch_merkle_layers(h->d_layers, N, s);
+  if (rc == ZP_OK && hipMemcpyAsync(h_root, h->d_layers + (2 * N - 2) * 32, 32,
+                                    hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "root copy failed");
+  if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
+  if (rc != ZP_OK) {
+    zp_hint_free(h);
+    return rc;
+  }
+  *hint_out = h;
+  return ZP_OK;
+}
+
+int zp_hint_info(const zp_hint* h, uint64_t* cw_bytes_per_poly, uint32_t* batch,
+                 uint64_t* num_leaves, uint32_t* height) {
+  if (!h) return fail(ZP_ERR_INVALID_PARAM, "null hint");
+  if (cw_bytes_per_poly) *cw_bytes_per_poly = h->per_poly;
+  if (batch) *batch = h->batch;
+  if (num_leaves) *num_leaves = h->N;
+  if (height) *height = h->height;
+  return ZP_OK;
+}
+
+const void* zp_hint_codewords(const zp_hint* h) { return h ? h->d_cw : nullptr; }
+
+int zp_hint_copy_codewords(const zp_hint* h, uint32_t poly, void* h_out, zp_stream stream) {
+  if (!h || poly >= h->batch) return fail(ZP_ERR_INVALID_PARAM, "bad hint or poly index");
+  hipStream_t s = (hipStream_t)stream;
+  ZP_HIP(hipMemcpyAsync(h_out, h->d_cw + poly * h->per_poly, h->per_poly, hipMemcpyDeviceToHost,
+                        s));
+  ZP_HIP(hipStreamSynchronize(s));
+  return ZP_OK;
+}
+
+int zp_hint_copy_layer(const zp_hint* h, uint32_t level, uint8_t* h_out, zp_stream stream) {
+  if (!h || level >= h->height) return fail(ZP_ERR_INVALID_PARAM, "bad hint or layer");
+  uint64_t off = 0, cnt = h->N;
+  for (uint32_t l = 0; l < level; l++) {
+    off += cnt;
+    cnt /= 2;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  ZP_HIP(hipMemcpyAsync(h_out, h->d_layers + off * 32, cnt * 32, hipMemcpyDeviceToHost, s));
+  ZP_HIP(hipStreamSynchronize(s));
+  return ZP_OK;
+}
+
+int zp_merkle_prove(const zp_hint* h, uint64_t leaf, uint8_t* h_siblings, uint32_t* h_len,
+                    zp_stream stream) {
+  if (!h || leaf >= h->N) return fail(ZP_ERR_INVALID_PARAM, "Invalid leaf index");
+  hipStream_t s = (hipStream_t)stream;
+  uint64_t off = 0, cnt = h->N, idx = leaf;
+  uint32_t k = 0;
+  while (cnt > 1) {
+    ZP_HIP(hipMemcpyAsync(h_siblings + 32 * k, h->d_layers + (off + (idx ^ 1)) * 32, 32,
+                          hipMemcpyDeviceToHost, s));
+    k++;
+    off += cnt;
+    cnt /= 2;
+    idx /= 2;
+  }
+  ZP_HIP(hipStreamSynchronize(s));
+  if (h_len) *h_len = k;
+  return ZP_OK;
+}
+
+int zp_hint_open_column(const zp_hint* h, uint64_t col, void* h_out, zp_stream stream) {
+  if (!h || col >= h->N) return fail(ZP_ERR_INVALID_PARAM, "bad hint or column");
+  hipStream_t s = (hipStream_t)stream;
+  const uint64_t rows = h->num_rows * h->batch;
+  ZP_HIP(hipMemcpy2DAsync(h_out, h->elem, h->d_cw + col * h->elem, h->N * h->elem, h->elem, rows,
+                          hipMemcpyDeviceToHost, s));
+  ZP_HIP(hipStreamSynchronize(s));
+  return ZP_OK;
+}
+
+void zp_hint_free(zp_hint* h) {
+  if (!h) return;
+  (void)hipFree(h->d_cw);
+  (void)hipFree(h->d_layers);
+  delete h;
+}
+
+int zp_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len, uint32_t batch,
+                    const uint64_t* d_evals, const int64_t* d_alphas, const int64_t* d_coeffs,
+                    uint64_t* d_out, zp_stream stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (D == 0 || D > 64) return fail(ZP_ERR_INVALID_PARAM, "degree bound must be within 1..64");
+  if (L < 3 || L > 8) return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
+  if (batch == 0 || num_rows == 0 || row_len == 0) return fail(ZP_ERR_SHAPE, "empty input");
+  void* scratch = nullptr;
+  ZP_HIP(hipMalloc(&scratch, combined_row_scratch_bytes(D, row_len, batch)));
+  int rc = launch_combined_row(D, L, num_rows, row_len, batch, d_evals, d_alphas, d_coeffs, d_out,
+                               scratch, s);
+  if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
+  (void)hipFree(scratch);
+  return rc;
+}
+
+}  // extern "C"

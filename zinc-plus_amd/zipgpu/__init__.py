@@ -1,0 +1,402 @@
+"""zipgpu -- the Zip/Zip+ commitment hot path on MI355X (gfx950).
+
+A host-side mirror of the reference crate zip-plus over the C ABI of
+include/zipgpu.h.  Names, argument meaning and error behaviour follow the
+reference:
+
+* ``IprsCode(row_len, depth)`` / ``IprsCode.new_with_optimal_depth(row_len)``
+  -- zip-plus/src/code/iprs.rs:30-51
+* ``RaaCode(row_len)`` -- zip-plus/src/code/raa.rs:38-90;
+  ``RaaSignFlippingCode(row_len)`` -- zip-plus/src/code/raa_sign_flip.rs
+* ``LinearCode.encode / row_len / codeword_len / params_string``
+  -- zip-plus/src/code.rs:10-72
+* ``ZipPlus.setup / commit / commit_single / encode_rows``
+  -- zip-plus/src/pcs/structs.rs:74-88, pcs/phase_commit.rs:61-176
+* ``ZipPlus.combined_row`` -- the combined-row block of ``prove_f``,
+  zip-plus/src/pcs/phase_prove.rs:153-244
+* ``MerkleTree`` root / ``prove`` / ``MerkleProof`` -- zip-plus/src/merkle.rs
+
+Device memory and streams come from torch (ROCm build); every computation
+runs in libzipgpu.so.  ``ZipError`` subclasses stand in for the reference's
+``Result::Err(ZipError::InvalidPcsParam)`` and its assert panics.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+__all__ = [
+    "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory",
+    "ZipTypes", "BinPolyTypes", "IntTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
+    "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
+    "merkle_root", "lib",
+]
+
+
+class ZipError(Exception):
+    """A non-zero zp_status from the library."""
+
+    def __init__(self, status, message):
+        super().__init__(f"{message} (status {status})")
+        self.status = status
+        self.message = message
+
+
+class InvalidPcsParam(ZipError):
+    """ZipError::InvalidPcsParam (zip-plus/src/lib.rs:13-29)."""
+
+
+class ShapeError(ZipError):
+    """The reference's assert! panics on shapes."""
+
+
+class DeviceError(ZipError):
+    """A HIP runtime failure."""
+
+
+class OutOfMemory(ZipError):
+    """Device allocation failed."""
+
+
+_ERRORS = {L.ZP_ERR_INVALID_PARAM: InvalidPcsParam, L.ZP_ERR_SHAPE: ShapeError,
+           L.ZP_ERR_DEVICE: DeviceError, L.ZP_ERR_OOM: OutOfMemory}
+
+
+def lib():
+    return L.load()
+
+
+def _check(rc):
+    if rc != L.ZP_OK:
+        raise _ERRORS.get(rc, ZipError)(rc, L.last_error())
+
+
+def _stream(stream=None):
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+def _dev_ptr(t):
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class ZipTypes:
+    """The Eval / Cw / CombR choice of a ZipTypes impl (pcs/structs.rs:15-60).
+
+    ``eval_type`` is "binpoly" (Eval = BinaryPoly<D>, u64 bit-packed, Cw =
+    [i64;D] under IPRS and [i32;D] under RAA) or "i32" (Eval = i32, Cw =
+    i128).  ``comb_limbs`` is L of CombR = Int<L>.
+    """
+
+    def __init__(self, eval_type, degree_plus_one=1, comb_limbs=5, num_column_openings=147):
+        if eval_type not in ("binpoly", "i32"):
+            raise ValueError(eval_type)
+        self.eval_type = eval_type
+        self.degree_plus_one = degree_plus_one if eval_type == "binpoly" else 1
+        self.comb_limbs = comb_limbs
+        self.num_column_openings = num_column_openings
+
+    @property
+    def code(self):
+        return L.ZP_EVAL_BINPOLY if self.eval_type == "binpoly" else L.ZP_EVAL_I32
+
+    @property
+    def eval_dtype(self):
+        return torch.int64 if self.eval_type == "binpoly" else torch.int32
+
+
+def BinPolyTypes(degree_plus_one, comb_limbs=5):
+    """BenchZipPlusTypes<_, D> (zip-plus/benches/zip_plus_benches.rs:37-65)."""
+    return ZipTypes("binpoly", degree_plus_one, comb_limbs)
+
+
+def IntTypes(comb_limbs=3):
+    """BenchZipTypes (zip-plus/benches/zip_benches.rs:25-40): i32 -> i128."""
+    return ZipTypes("i32", 1, comb_limbs)
+
+
+class LinearCode:
+    """A code with device-resident tables (trait LinearCode, code.rs:10-72)."""
+
+    KIND = None
+
+    def __init__(self, kind, rep, row_len, depth):
+        self._h = None
+        lib_ = L.load()
+        h = ctypes.c_void_p()
+        _check(lib_.zp_code_create(kind, rep, row_len, depth, ctypes.byref(h)))
+        self._h = h
+        self._lib = lib_
+        self.REPETITION_FACTOR = rep
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.zp_code_destroy(self._h)
+            self._h = None
+
+    def row_len(self):
+        return int(self._lib.zp_code_row_len(self._h))
+
+    def codeword_len(self):
+        return int(self._lib.zp_code_codeword_len(self._h))
+
+    def cw_shape(self, types):
+        """Trailing shape and dtype of one codeword element."""
+        if self.KIND == L.ZP_CODE_IPRS:
+            if types.eval_type == "i32":
+                return (2,), torch.int64  # i128 as two LE words
+            return (types.degree_plus_one,), torch.int64
+        return (types.degree_plus_one,), torch.int32
+
+    def cw_elem_bytes(self, types):
+        return int(self._lib.zp_cw_elem_bytes(self._h, types.code, types.degree_plus_one))
+
+    def encode_rows(self, evals, num_rows, types, out=None, stream=None):
+        """Encode num_rows consecutive rows of a flat device eval tensor."""
+        if evals.dtype != types.eval_dtype:
+            raise ValueError(f"evals must be {types.eval_dtype}")
+        if evals.numel() != num_rows * self.row_len():
+            raise ShapeError(L.ZP_ERR_SHAPE,
+                             f"Input length {evals.numel()} does not match "
+                             f"{num_rows} rows of {self.row_len()}")
+        tail, dtype = self.cw_shape(types)
+        if out is None:
+            out = torch.empty((num_rows, self.codeword_len()) + tail, dtype=dtype,
+                              device=evals.device)
+        _check(self._lib.zp_encode_rows(self._h, types.code, types.degree_plus_one,
+                                        _dev_ptr(evals), num_rows, _dev_ptr(out),
+                                        _stream(stream)))
+        return out
+
+    def encode(self, row, types):
+        """LinearCode::encode of one row."""
+        if row.numel() != self.row_len():
+            raise ShapeError(L.ZP_ERR_SHAPE,
+                             f"Input length {row.numel()} does not match expected row length "
+                             f"{self.row_len()}")
+        return self.encode_rows(row.reshape(-1), 1, types)[0]
+
+
+class IprsCode(LinearCode):
+    """IprsCode<Zt, PnttConfigF65537, REP, _> (zip-plus/src/code/iprs.rs)."""
+
+    KIND = L.ZP_CODE_IPRS
+
+    def __init__(self, row_len, depth, rep=4):
+        super().__init__(L.ZP_CODE_IPRS, rep, row_len, depth)
+        self.depth = int(self._lib.zp_code_depth(self._h))
+
+    @classmethod
+    def new_with_optimal_depth(cls, row_len, rep=4):
+        return cls(row_len, -1, rep)
+
+    def params_string(self):
+        return f"row_len={self.row_len()}, rate=1/{self.REPETITION_FACTOR}, depth={self.depth}"
+
+
+class RaaCode(LinearCode):
+    """RaaCode<Zt, _, REP> with permutation seeds 1 and 2 (raa.rs:38-90)."""
+
+    KIND = L.ZP_CODE_RAA
+
+    def __init__(self, row_len, rep=4):
+        super().__init__(self.KIND, rep, row_len, 0)
+
+    def params_string(self):
+        return f"row_len={self.row_len()}, rate=1/{self.REPETITION_FACTOR}"
+
+
+class RaaSignFlippingCode(RaaCode):
+    """RaaSignFlippingCode (zip-plus/src/code/raa_sign_flip.rs)."""
+
+    KIND = L.ZP_CODE_RAA_SIGNFLIP
+
+
+class ZipPlusParams:
+    """ZipPlusParams (pcs/structs.rs:91-109)."""
+
+    def __init__(self, num_vars, num_rows, linear_code):
+        self.num_vars = num_vars
+        self.num_rows = num_rows
+        self.linear_code = linear_code
+
+
+class ZipPlusCommitment:
+    """ZipPlusCommitment { root, batch_size } (pcs/structs.rs:131-157)."""
+
+    def __init__(self, root, batch_size):
+        self.root = root
+        self.batch_size = batch_size
+
+    def __eq__(self, other):
+        return self.root == other.root and self.batch_size == other.batch_size
+
+    def to_bytes(self):
+        """Transcription bytes: root then batch_size as u64 LE (structs.rs:140-153)."""
+        return self.root + int(self.batch_size).to_bytes(8, "little")
+
+
+class MerkleProof:
+    """MerkleProof { leaf_index, leaf_count, siblings } (merkle.rs:259-278)."""
+
+    def __init__(self, leaf_index, leaf_count, siblings):
+        self.leaf_index = leaf_index
+        self.leaf_count = leaf_count
+        self.siblings = siblings
+
+
+class ZipPlusHint:
+    """ZipPlusHint kept on the device: codewords plus every Merkle layer."""
+
+    def __init__(self, handle, types, code, num_rows):
+        self._h = handle
+        self._lib = L.load()
+        self.types = types
+        self.code = code
+        self.num_rows = num_rows
+        per_poly, batch, leaves, height = ctypes.c_uint64(), ctypes.c_uint32(), \
+            ctypes.c_uint64(), ctypes.c_uint32()
+        _check(self._lib.zp_hint_info(handle, ctypes.byref(per_poly), ctypes.byref(batch),
+                                      ctypes.byref(leaves), ctypes.byref(height)))
+        self.cw_bytes_per_poly = per_poly.value
+        self.batch = batch.value
+        self.num_leaves = leaves.value
+        self.height = height.value
+
+    def __del__(self):
+        self.free()
+
+    def free(self):
+        if getattr(self, "_h", None):
+            self._lib.zp_hint_free(self._h)
+            self._h = None
+
+    def _cw_np(self):
+        tail, dtype = self.code.cw_shape(self.types)
+        return (self.num_rows, self.code.codeword_len()) + tail, \
+            np.int64 if dtype == torch.int64 else np.int32
+
+    def codewords(self, poly=0, stream=None):
+        """The poly's DenseRowMatrix<Cw> as a host array [rows, N, ...]."""
+        shape, dt = self._cw_np()
+        out = np.empty(shape, dtype=dt)
+        _check(self._lib.zp_hint_copy_codewords(self._h, poly, out.ctypes.data_as(ctypes.c_void_p),
+                                                _stream(stream)))
+        return out
+
+    def layer(self, level, stream=None):
+        n = self.num_leaves >> level
+        out = np.empty((n, 32), dtype=np.uint8)
+        _check(self._lib.zp_hint_copy_layer(self._h, level, out.ctypes.data_as(ctypes.c_void_p),
+                                            _stream(stream)))
+        return out
+
+    def prove(self, leaf_index, stream=None):
+        """MerkleTree::prove (merkle.rs:105-120)."""
+        buf = np.empty((64, 32), dtype=np.uint8)
+        n = ctypes.c_uint32()
+        _check(self._lib.zp_merkle_prove(self._h, leaf_index, buf.ctypes.data_as(ctypes.c_void_p),
+                                         ctypes.byref(n), _stream(stream)))
+        return MerkleProof(leaf_index, self.num_leaves, [bytes(buf[i]) for i in range(n.value)])
+
+    def open_column(self, col, stream=None):
+        """The bytes of column `col` over all rows of all polys."""
+        nbytes = self.batch * self.num_rows * self.code.cw_elem_bytes(self.types)
+        out = np.empty(nbytes, dtype=np.uint8)
+        _check(self._lib.zp_hint_open_column(self._h, col, out.ctypes.data_as(ctypes.c_void_p),
+                                             _stream(stream)))
+        return out.tobytes()
+
+
+def merkle_root(rows, elem_bytes, stream=None):
+    """MerkleTree::new(rows).root() over a device tensor [num_rows, width, ...]."""
+    if rows.dim() < 2:
+        raise ShapeError(L.ZP_ERR_SHAPE, "expected [num_rows, width, ...]")
+    out = (ctypes.c_uint8 * 32)()
+    _check(L.load().zp_merkle_root(_dev_ptr(rows), rows.shape[0], rows.shape[1], elem_bytes, out,
+                                   _stream(stream)))
+    return bytes(out)
+
+
+class ZipPlus:
+    """ZipPlus<Zt, Lc> (pcs/structs.rs:66-89, phase_commit.rs, phase_prove.rs)."""
+
+    def __init__(self, types):
+        self.types = types
+
+    @staticmethod
+    def setup(poly_size, linear_code):
+        """ZipPlus::setup (pcs/structs.rs:74-88)."""
+        if poly_size <= 0 or poly_size & (poly_size - 1):
+            raise ShapeError(L.ZP_ERR_SHAPE, "poly_size must be a power of two")
+        row_len = linear_code.row_len()
+        if row_len <= 0 or poly_size % row_len:
+            raise ShapeError(L.ZP_ERR_SHAPE,
+                             f"poly_size ({poly_size}) must be divisible by row_len ({row_len})")
+        num_rows = poly_size // row_len
+        if num_rows & (num_rows - 1):
+            raise ShapeError(L.ZP_ERR_SHAPE, f"num_rows ({num_rows}) must be a power of two")
+        return ZipPlusParams(poly_size.bit_length() - 1, num_rows, linear_code)
+
+    def _stack(self, pp, polys):
+        if isinstance(polys, torch.Tensor):
+            t = polys if polys.dim() == 2 else polys.reshape(1, -1)
+        else:
+            if len(polys) == 0:
+                raise ShapeError(L.ZP_ERR_SHAPE, "Batch must contain at least one polynomial")
+            t = torch.stack([p.reshape(-1) for p in polys])
+        if t.shape[0] == 0:
+            raise ShapeError(L.ZP_ERR_SHAPE, "Batch must contain at least one polynomial")
+        expected = pp.num_rows * pp.linear_code.row_len()
+        if t.shape[1] != expected:
+            raise ShapeError(L.ZP_ERR_SHAPE,
+                             f"Polynomial has an incorrect number of evaluations ({t.shape[1]}) "
+                             f"for the expected matrix size ({expected})")
+        if t.dtype != self.types.eval_dtype:
+            raise ValueError(f"evals must be {self.types.eval_dtype}")
+        return t.contiguous()
+
+    def encode_rows(self, pp, evals, stream=None):
+        """ZipPlus::encode_rows (phase_commit.rs:159-176)."""
+        return pp.linear_code.encode_rows(evals.reshape(-1), pp.num_rows, self.types,
+                                          stream=stream)
+
+    def commit(self, pp, polys, stream=None):
+        """ZipPlus::commit (phase_commit.rs:61-101) -> (hint, commitment)."""
+        t = self._stack(pp, polys)
+        h = ctypes.c_void_p()
+        root = (ctypes.c_uint8 * 32)()
+        _check(L.load().zp_commit(pp.linear_code._h, self.types.code, self.types.degree_plus_one,
+                                  _dev_ptr(t), pp.num_rows, t.shape[0], ctypes.byref(h), root,
+                                  _stream(stream)))
+        hint = ZipPlusHint(h, self.types, pp.linear_code, pp.num_rows)
+        return hint, ZipPlusCommitment(bytes(root), t.shape[0])
+
+    def commit_single(self, pp, poly, stream=None):
+        return self.commit(pp, [poly], stream)
+
+    def combined_row(self, pp, polys, alphas, coeffs, stream=None):
+        """The combined row of prove_f (phase_prove.rs:153-244).
+
+        alphas: device int64 [batch, D, 2] (i128 LE words), coeffs: device
+        int64 [num_rows, 2]; returns int64 [row_len, L] (Int<L> limbs).
+        """
+        if self.types.eval_type != "binpoly":
+            raise InvalidPcsParam(L.ZP_ERR_INVALID_PARAM, "combined row needs binary polys")
+        t = self._stack(pp, polys)
+        D, Lm = self.types.degree_plus_one, self.types.comb_limbs
+        row_len = pp.linear_code.row_len()
+        if alphas.numel() != t.shape[0] * D * 2 or coeffs.numel() != pp.num_rows * 2:
+            raise ShapeError(L.ZP_ERR_SHAPE, "alphas/coeffs have the wrong length")
+        out = torch.empty((row_len, Lm), dtype=torch.int64, device=t.device)
+        _check(L.load().zp_combined_row(D, Lm, pp.num_rows, row_len, t.shape[0], _dev_ptr(t),
+                                        _dev_ptr(alphas.contiguous()),
+                                        _dev_ptr(coeffs.contiguous()), _dev_ptr(out),
+                                        _stream(stream)))
+        return out

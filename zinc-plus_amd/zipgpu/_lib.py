@@ -1,0 +1,83 @@
+"""ctypes binding of libzipgpu.so (the C ABI of include/zipgpu.h).
+
+The library is loaded from the package's parent directory, where the build
+puts it.  There is no fallback: if the HIP library is missing, every call
+raises, so a GPU run can never silently route around the device code.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libzipgpu.so")
+
+ZP_OK = 0
+ZP_ERR_INVALID_PARAM = 1
+ZP_ERR_OVERFLOW = 2
+ZP_ERR_SHAPE = 3
+ZP_ERR_DEVICE = 4
+ZP_ERR_OOM = 5
+
+ZP_CODE_IPRS = 0
+ZP_CODE_RAA = 1
+ZP_CODE_RAA_SIGNFLIP = 2
+
+ZP_EVAL_I32 = 0
+ZP_EVAL_BINPOLY = 1
+
+_c_void_p = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_i32 = ctypes.c_int32
+_int = ctypes.c_int
+
+# name -> (restype, argtypes); mirrors include/zipgpu.h one for one.
+SIGNATURES = {
+    "zp_init": (_int, [_int]),
+    "zp_status_string": (ctypes.c_char_p, [_int]),
+    "zp_last_error": (ctypes.c_char_p, []),
+    "zp_code_create": (_int, [_int, _u32, _u64, _i32, ctypes.POINTER(_c_void_p)]),
+    "zp_code_destroy": (None, [_c_void_p]),
+    "zp_code_row_len": (_u64, [_c_void_p]),
+    "zp_code_codeword_len": (_u64, [_c_void_p]),
+    "zp_code_depth": (_u32, [_c_void_p]),
+    "zp_code_kind_of": (_int, [_c_void_p]),
+    "zp_cw_elem_bytes": (_u64, [_c_void_p, _int, _u32]),
+    "zp_encode_rows": (_int, [_c_void_p, _int, _u32, _c_void_p, _u64, _c_void_p, _c_void_p]),
+    "zp_merkle_root": (_int, [_c_void_p, _u64, _u64, _u32, _c_void_p, _c_void_p]),
+    "zp_commit": (_int, [_c_void_p, _int, _u32, _c_void_p, _u64, _u32,
+                         ctypes.POINTER(_c_void_p), _c_void_p, _c_void_p]),
+    "zp_hint_info": (_int, [_c_void_p, ctypes.POINTER(_u64), ctypes.POINTER(_u32),
+                            ctypes.POINTER(_u64), ctypes.POINTER(_u32)]),
+    "zp_hint_codewords": (_c_void_p, [_c_void_p]),
+    "zp_hint_copy_codewords": (_int, [_c_void_p, _u32, _c_void_p, _c_void_p]),
+    "zp_hint_copy_layer": (_int, [_c_void_p, _u32, _c_void_p, _c_void_p]),
+    "zp_merkle_prove": (_int, [_c_void_p, _u64, _c_void_p, ctypes.POINTER(_u32), _c_void_p]),
+    "zp_hint_open_column": (_int, [_c_void_p, _u64, _c_void_p, _c_void_p]),
+    "zp_hint_free": (None, [_c_void_p]),
+    "zp_combined_row": (_int, [_u32, _u32, _u64, _u64, _u32, _c_void_p, _c_void_p, _c_void_p,
+                               _c_void_p, _c_void_p]),
+}
+
+_lib = None
+
+
+def load():
+    """Load libzipgpu.so once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"libzipgpu.so not found at {LIB_PATH}: build it first "
+            "(python -c 'import __graft_entry__ as g; g.build()')")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def last_error():
+    return load().zp_last_error().decode(errors="replace")
