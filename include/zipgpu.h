@@ -61,6 +61,12 @@ int zp_init(int device);
 const char* zp_status_string(int status);
 const char* zp_last_error(void);
 
+/* Library options (no reference counterpart):
+ *   "iprs_encoder": 0 = matrix-core encoder where eligible (default),
+ *                   1 = VALU encoder for every shape.
+ * Returns ZP_ERR_INVALID_PARAM for an unknown name. */
+int zp_set_option(const char* name, int64_t value);
+
 /* ---- LinearCode (zip-plus/src/code.rs:10-72) -------------------------- */
 
 /* IprsCode::new(row_len, depth) (zip-plus/src/code/iprs.rs:30-37; errors as

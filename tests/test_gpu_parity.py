@@ -49,6 +49,27 @@ def test_iprs_binary_ignores_bits_above_degree(zg, dev, oracle):
     assert np.array_equal(got, oracle.iprs_encode_binary(x.view(np.uint64), 1, row_len, -1, D)[0])
 
 
+@pytest.mark.parametrize("row_len,depth,D", [
+    (256, 1, 32), (2048, 1, 64), (1 << 13, 2, 32), (1 << 14, 2, 64), (1 << 12, 2, 16)])
+def test_iprs_device_encoders_agree(zg, dev, oracle, row_len, depth, D):
+    # The matrix-core encoder and the VALU encoder are separate device paths;
+    # both must equal the oracle.
+    say("encoders", row_len, depth, D)
+    rows = 2
+    x = synth.binary_polys(600 + row_len + D, rows * row_len, D)
+    code = zg.IprsCode(row_len, depth)
+    types = zg.BinPolyTypes(D)
+    xd = to_dev(x, dev)
+    fast = code.encode_rows(xd, rows, types).cpu().numpy()
+    zg.set_option("iprs_encoder", 1)
+    try:
+        valu = code.encode_rows(xd, rows, types).cpu().numpy()
+    finally:
+        zg.set_option("iprs_encoder", 0)
+    want = oracle.iprs_encode_binary(x.view(np.uint64), rows, row_len, depth, D)
+    assert np.array_equal(fast, want) and np.array_equal(valu, want)
+
+
 @pytest.mark.parametrize("row_len,depth,rows", [
     (256, 1, 2), (1 << 14, 2, 4), (2048, 3, 1), (128, 0, 1), (1 << 12, -1, 3)])
 def test_iprs_i32_encode(zg, dev, oracle, row_len, depth, rows):
@@ -127,7 +148,7 @@ def test_raa_known_answer_pattern(zg, dev, oracle):
 
 @pytest.mark.parametrize("rows,width,elem", [
     (1, 1, 4), (4, 16, 16), (64, 1024, 256), (5, 64, 256), (3, 8, 24), (2048, 64, 256),
-    (1, 4, 8), (17, 32, 12)])
+    (1, 4, 8), (17, 32, 12), (3, 16, 1024), (12, 8, 512), (7, 4, 320), (4, 2, 64)])
 def test_merkle_root(zg, dev, oracle, rows, width, elem):
     say("merkle", rows, width, elem)
     m = synth.splitmix64(rows * 1000 + width, rows * width * elem // 8 + 1).view(np.uint8)

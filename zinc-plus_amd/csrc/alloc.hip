@@ -1,0 +1,113 @@
+// alloc.hip -- device memory cache for hints and transient scratch.
+//
+// Blocks are cached per (device, stream).  A block freed on stream s is only
+// handed out again to work on the same stream s, so stream order makes the
+// reuse safe without events or synchronisation.  Sizes are rounded up (2 MiB
+// granules above 1 MiB) so that repeated commits of one shape hit the cache.
+// When hipMalloc fails, the cache of that device is released (after a device
+// synchronisation) and the allocation is retried once.
+#include <map>
+#include <mutex>
+#include <unordered_map>
+#include <utility>
+
+#include "common.h"
+
+namespace zpi {
+namespace {
+
+struct Key {
+  int dev;
+  hipStream_t stream;
+  bool operator<(const Key& o) const {
+    return dev != o.dev ? dev < o.dev : (uintptr_t)stream < (uintptr_t)o.stream;
+  }
+};
+
+std::mutex g_mu;
+std::map<Key, std::multimap<size_t, void*>> g_free;
+std::unordered_map<void*, std::pair<int, size_t>> g_size;  // every cached or live block
+
+size_t round_size(size_t n) {
+  if (n == 0) n = 1;
+  const size_t g = n >= (1u << 20) ? (2u << 20) : 512;
+  return (n + g - 1) / g * g;
+}
+
+void release_device_locked(int dev) {
+  (void)hipDeviceSynchronize();
+  for (auto it = g_free.begin(); it != g_free.end();) {
+    if (it->first.dev != dev) {
+      ++it;
+      continue;
+    }
+    for (auto& kv : it->second) {
+      (void)hipFree(kv.second);
+      g_size.erase(kv.second);
+    }
+    it = g_free.erase(it);
+  }
+}
+
+}  // namespace
+
+int dev_alloc(void** out, size_t bytes, hipStream_t s) {
+  *out = nullptr;
+  int dev = 0;
+  ZP_HIP(hipGetDevice(&dev));
+  const size_t want = round_size(bytes);
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto& fl = g_free[Key{dev, s}];
+  auto it = fl.lower_bound(want);
+  if (it != fl.end() && it->first <= 2 * want) {
+    *out = it->second;
+    fl.erase(it);
+    return ZP_OK;
+  }
+  void* p = nullptr;
+  if (hipMalloc(&p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    release_device_locked(dev);
+    if (hipMalloc(&p, want) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(ZP_ERR_OOM, "device allocation of " + std::to_string(want) + " bytes failed");
+    }
+  }
+  g_size[p] = {dev, want};
+  *out = p;
+  return ZP_OK;
+}
+
+void dev_free(void* p, hipStream_t s) {
+  if (!p) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_size.find(p);
+  if (it == g_size.end()) return;
+  g_free[Key{it->second.first, s}].emplace(it->second.second, p);
+}
+
+void dev_release_cache() {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return;
+  std::lock_guard<std::mutex> lk(g_mu);
+  release_device_locked(dev);
+}
+
+// A per-thread pinned staging buffer for small device-to-host results.
+uint8_t* pinned_scratch(size_t bytes) {
+  static thread_local uint8_t* buf = nullptr;
+  static thread_local size_t cap = 0;
+  if (cap < bytes) {
+    if (buf) (void)hipHostFree(buf);
+    buf = nullptr;
+    cap = 0;
+    if (hipHostMalloc((void**)&buf, bytes, hipHostMallocDefault) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    cap = bytes;
+  }
+  return buf;
+}
+
+}  // namespace zpi

@@ -20,6 +20,7 @@ using namespace zpi;
 
 struct zp_hint {
   int device = 0;
+  hipStream_t stream = nullptr;  // the commit's stream; buffers are freed on it
   uint8_t* d_cw = nullptr;      // batch x num_rows x N elements, poly-major
   uint8_t* d_layers = nullptr;  // (2N-1) x 32 bytes, leaves first, root last
   uint64_t N = 0, num_rows = 0, elem = 0, per_poly = 0;
@@ -29,6 +30,10 @@ struct zp_hint {
 namespace {
 
 bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
+
+// Selects the VALU IPRS encoder instead of the matrix-core one (set by
+// zp_set_option; used to cross-check the two device paths).
+int g_force_valu = 0;
 
 template <class T>
 int upload(const std::vector<T>& host, T** dev) {
@@ -41,6 +46,9 @@ void free_code(zp_code* c) {
   if (!c) return;
   (void)hipFree(c->d_ftab);
   for (auto* p : c->d_tw) (void)hipFree(p);
+  (void)hipFree(c->d_afrag);
+  (void)hipFree(c->d_tw0f);
+  (void)hipFree(c->d_tw1f);
   (void)hipFree(c->d_perm1);
   (void)hipFree(c->d_perm2);
   delete c;
@@ -76,8 +84,11 @@ int encode(const zp_code* c, int eval_type, uint32_t D, const void* evals, uint6
            void* out, hipStream_t s) {
   ZP_TRY(check_types(c, eval_type, D));
   if (c->kind == ZP_CODE_IPRS) {
-    if (eval_type == ZP_EVAL_BINPOLY)
+    if (eval_type == ZP_EVAL_BINPOLY) {
+      if (iprs_mfma_eligible(c, D) && !g_force_valu)
+        return launch_iprs_binary_mfma(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
       return launch_iprs_binary(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
+    }
     return launch_iprs_i32(c, (const int32_t*)evals, num_rows, out, s);
   }
   return launch_raa_binary(c, (const uint64_t*)evals, num_rows, D, (int32_t*)out, s);
@@ -108,6 +119,16 @@ const char* zp_status_string(int status) {
 
 const char* zp_last_error(void) { return g_last_error.c_str(); }
 
+int zp_set_option(const char* name, int64_t value) {
+  if (!name) return fail(ZP_ERR_INVALID_PARAM, "null option name");
+  if (std::strcmp(name, "iprs_encoder") == 0) {
+    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "iprs_encoder is 0 or 1");
+    g_force_valu = (int)value;
+    return ZP_OK;
+  }
+  return fail(ZP_ERR_INVALID_PARAM, std::string("unknown option ") + name);
+}
+
 int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_code** out) {
   if (!out) return fail(ZP_ERR_INVALID_PARAM, "null output");
   *out = nullptr;
@@ -132,10 +153,25 @@ int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_c
       std::vector<int32_t> f(bd);
       iprs_ftab(bd, f.data());
       rc = upload(f, &c->d_ftab);
+      std::vector<std::vector<int32_t>> tws(c->depth);
       for (uint32_t k = 0; rc == ZP_OK && k < c->depth; k++) {
-        std::vector<int32_t> tw(((uint64_t)bd << (3 * k)) * 56);
-        iprs_twiddles(bd, c->N, c->depth, k, tw.data());
-        rc = upload(tw, &c->d_tw[k]);
+        tws[k].resize(((uint64_t)bd << (3 * k)) * 56);
+        iprs_twiddles(bd, c->N, c->depth, k, tws[k].data());
+        rc = upload(tws[k], &c->d_tw[k]);
+      }
+      // Tables of the matrix-core encoder (depth 1 or 2, K up to 256).
+      const uint32_t ksteps = (bl + 63) / 64;
+      if (rc == ZP_OK && (c->depth == 1 || c->depth == 2) && bd >= 16 && ksteps <= 4) {
+        std::vector<int8_t> af((uint64_t)(bd / 16) * ksteps * 3 * 64 * 16);
+        iprs_afrag(bd, bl, ksteps, f.data(), af.data());
+        rc = upload(af, (int8_t**)&c->d_afrag);
+        std::vector<double> t0(tws[0].begin(), tws[0].end());
+        if (rc == ZP_OK) rc = upload(t0, &c->d_tw0f);
+        if (rc == ZP_OK && c->depth == 2) {
+          std::vector<double> t1(tws[1].begin(), tws[1].end());
+          rc = upload(t1, &c->d_tw1f);
+        }
+        if (rc == ZP_OK) c->mfma_ksteps = ksteps;
       }
     }
   } else if (kind == ZP_CODE_RAA || kind == ZP_CODE_RAA_SIGNFLIP) {
@@ -183,15 +219,21 @@ int zp_merkle_root(const void* d_rows, uint64_t num_rows, uint64_t width, uint32
   if (num_rows == 0 || width == 0) return fail(ZP_ERR_SHAPE, "empty matrix");
   if (!pow2(width)) return fail(ZP_ERR_SHAPE, "row width must be a power of two");
   uint8_t* layers = nullptr;
-  ZP_HIP(hipMalloc((void**)&layers, (2 * width - 1) * 32));
+  ZP_TRY(dev_alloc((void**)&layers, (2 * width - 1) * 32, s));
+  uint8_t* pin = pinned_scratch(32);
+  if (!pin) {
+    dev_free(layers, s);
+    return fail(ZP_ERR_OOM, "pinned staging allocation failed");
+  }
   int rc = launch_leaf_hash((const uint8_t*)d_rows, num_rows, width, elem, layers, s);
   if (rc == ZP_OK) rc = launch_merkle_layers(layers, width, s);
-  if (rc == ZP_OK && hipMemcpyAsync(h_root, layers + (2 * width - 2) * 32, 32,
+  if (rc == ZP_OK && hipMemcpyAsync(pin, layers + (2 * width - 2) * 32, 32,
                                     hipMemcpyDeviceToHost, s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "root copy failed");
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
-  (void)hipFree(layers);
+  if (rc == ZP_OK) std::memcpy(h_root, pin, 32);
+  dev_free(layers, s);
   return rc;
 }
 
@@ -214,12 +256,11 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_eval
   h->per_poly = num_rows * N * elem;
   h->batch = batch;
   h->height = (uint32_t)__builtin_ctzll(N) + 1;
-  int rc = ZP_OK;
-  if (hipMalloc((void**)&h->d_cw, h->per_poly * batch) != hipSuccess ||
-      hipMalloc((void**)&h->d_layers, (2 * N - 1) * 32) != hipSuccess) {
-    (void)hipGetLastError();
-    rc = fail(ZP_ERR_OOM, "device allocation for the commitment hint failed");
-  }
+  h->stream = s;
+  int rc = dev_alloc((void**)&h->d_cw, h->per_poly * batch, s);
+  if (rc == ZP_OK) rc = dev_alloc((void**)&h->d_layers, (2 * N - 1) * 32, s);
+  uint8_t* pin = pinned_scratch(32);
+  if (rc == ZP_OK && !pin) rc = fail(ZP_ERR_OOM, "pinned staging allocation failed");
   const uint64_t in_stride = num_rows * code->row_len * eval_bytes(eval_type);
   for (uint32_t b = 0; rc == ZP_OK && b < batch; b++)
     rc = encode(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride, num_rows,
@@ -227,11 +268,12 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_eval
   if (rc == ZP_OK) rc = launch_leaf_hash(h->d_cw, num_rows * batch, N, (uint32_t)elem,
                                          h->d_layers, s);
   if (rc == ZP_OK) rc = launch_merkle_layers(h->d_layers, N, s);
-  if (rc == ZP_OK && hipMemcpyAsync(h_root, h->d_layers + (2 * N - 2) * 32, 32,
+  if (rc == ZP_OK && hipMemcpyAsync(pin, h->d_layers + (2 * N - 2) * 32, 32,
                                     hipMemcpyDeviceToHost, s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "root copy failed");
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
+  if (rc == ZP_OK) std::memcpy(h_root, pin, 32);
   if (rc != ZP_OK) {
     zp_hint_free(h);
     return rc;
@@ -305,8 +347,8 @@ int zp_hint_open_column(const zp_hint* h, uint64_t col, void* h_out, zp_stream s
 
 void zp_hint_free(zp_hint* h) {
   if (!h) return;
-  (void)hipFree(h->d_cw);
-  (void)hipFree(h->d_layers);
+  dev_free(h->d_cw, h->stream);
+  dev_free(h->d_layers, h->stream);
   delete h;
 }
 
@@ -318,12 +360,12 @@ int zp_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
   if (L < 3 || L > 8) return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
   if (batch == 0 || num_rows == 0 || row_len == 0) return fail(ZP_ERR_SHAPE, "empty input");
   void* scratch = nullptr;
-  ZP_HIP(hipMalloc(&scratch, combined_row_scratch_bytes(D, row_len, batch)));
+  ZP_TRY(dev_alloc(&scratch, combined_row_scratch_bytes(D, row_len, batch), s));
   int rc = launch_combined_row(D, L, num_rows, row_len, batch, d_evals, d_alphas, d_coeffs, d_out,
                                scratch, s);
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
-  (void)hipFree(scratch);
+  dev_free(scratch, s);
   return rc;
 }
 

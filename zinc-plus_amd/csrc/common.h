@@ -42,6 +42,12 @@ struct zp_code {
   uint32_t depth = 0, bl = 0, bd = 0, bd_log2 = 0;
   int32_t* d_ftab = nullptr;                         // f(e) = norm(w_bd^e), e < bd
   int32_t* d_tw[zpi::kMaxIprsDepth] = {nullptr, nullptr, nullptr};  // [bd*8^k][7][8]
+  // MFMA encoder tables (depth 1 or 2): int8 slices of the base matrix in
+  // the v_mfma_i32_16x16x64_i8 operand layout, and stage twiddles as double.
+  void* d_afrag = nullptr;    // [bd/16][ksteps][3 slices][64 lanes][16 bytes]
+  double* d_tw0f = nullptr;   // stage 0, [bd][7][8]
+  double* d_tw1f = nullptr;   // stage 1, [8*bd][7][8] (depth 2)
+  uint32_t mfma_ksteps = 0;   // ceil(bl / 64)
   // RAA
   uint32_t* d_perm1 = nullptr;
   uint32_t* d_perm2 = nullptr;
@@ -49,16 +55,27 @@ struct zp_code {
 
 namespace zpi {
 
+// Device memory cache (alloc.hip): blocks are reused only on the stream
+// that freed them.
+int dev_alloc(void** out, size_t bytes, hipStream_t s);
+void dev_free(void* p, hipStream_t s);
+void dev_release_cache();
+uint8_t* pinned_scratch(size_t bytes);
+
 // Host-side tables (params.cpp).
 int iprs_params(uint64_t row_len, uint64_t depth, uint64_t rep, uint32_t* bl, uint32_t* bd);
 uint64_t iprs_optimal_depth(uint64_t row_len);
 void iprs_ftab(uint32_t bd, int32_t* out);
 void iprs_twiddles(uint32_t bd, uint64_t N, uint32_t depth, uint32_t stage, int32_t* out);
 void raa_perm(uint64_t n, uint64_t seed, uint32_t* out);
+void iprs_afrag(uint32_t bd, uint32_t bl, uint32_t ksteps, const int32_t* ftab, int8_t* out);
 
 // Kernel launchers (asynchronous on `s`).
 int launch_iprs_binary(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
                        int64_t* out, hipStream_t s);
+bool iprs_mfma_eligible(const zp_code* c, uint32_t D);
+int launch_iprs_binary_mfma(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
+                            int64_t* out, hipStream_t s);
 int launch_iprs_i32(const zp_code* c, const int32_t* evals, uint64_t rows, void* out,
                     hipStream_t s);
 int launch_raa_binary(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,

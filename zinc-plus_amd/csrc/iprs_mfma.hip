@@ -1,0 +1,280 @@
+// iprs_mfma.hip -- IPRS encoder for binary polynomials with the base layer
+// on the int8 matrix cores of gfx950.
+//
+// Reference: zip-plus/src/code/iprs/pntt/radix8.rs:117-161 (base layer) and
+// :58-113 with butterfly.rs:24-43 (radix-8 stages).
+//
+// The base layer of one row is an exact integer product
+//   B[r][(c,p)] = sum_k M[r][k] * bit_p(x[octrev(c) + k*8^d]),
+// M[r][k] = f(r*k mod bd) in [-32769, 32767].  Each f is split into three
+// int8 slices f = s0 + 128*s1 + 16384*s2 (s0, s1 in [0,127], s2 in [-3,1]),
+// and every slice product is one v_mfma_i32_16x16x64_i8 whose partial sums
+// stay below 2^15: the recombined i32 sum is exact (|B| < 2^23).
+//
+// The radix-8 stages run in FP64 FMA.  With |B| < 2^23 and |tw| <= 32769,
+// stage-0 products and sums stay below 2^42, so they are exact.  Stage-1
+// inputs (< 2^42) are split at 2^21 so every product and partial sum stays
+// below 2^40, again exact; the two halves are recombined in int64.  The
+// results equal the reference's wrapping-i64 arithmetic bit for bit because
+// no value on this path exceeds 2^59.
+//
+// Work split: a pre-pass writes, per row and chunk c, the 16-bit words
+// P[c][w][p] = bits of plane p for k = 16w .. 16w+15.  The main kernel owns
+// (row, 16 base rows r, 16 planes p) and all 8^d chunks: MFMA tiles are
+// (16 r) x (16 planes) for one chunk, the summed base values go to LDS, and
+// each thread then runs every stage of one (r, p) column and writes its
+// outputs; the 16 planes of one (c, r) are 128 contiguous bytes.
+#include "common.h"
+
+namespace zpi {
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t octrev_d(uint32_t c, uint32_t d) {
+  uint32_t r = 0;
+  for (uint32_t i = 0; i < d; i++) {
+    r = (r << 3) | (c & 7u);
+    c >>= 3;
+  }
+  return r;
+}
+
+// Exact conversion of an integer-valued double with |x| < 2^51.
+__device__ __forceinline__ int64_t dbl_to_i64(double x) {
+  const double magic = 6755399441055744.0;  // 1.5 * 2^52
+  return __double_as_longlong(x + magic) - __double_as_longlong(magic);
+}
+
+// Bits 0..3 of `nib` -> bytes 0..3 (each 0 or 1).
+__device__ __forceinline__ int spread4(uint32_t nib) {
+  return (int)((nib * 0x00204081u) & 0x01010101u);
+}
+
+// planes[(row*NC + c)*W + w][p] = sum_i bit_p(x[octrev(c) + NC*(16w+i)]) << i
+template <int NC>
+__global__ void __launch_bounds__(256)
+bitplane_kernel(const uint64_t* __restrict__ evals, uint16_t* __restrict__ planes,
+                uint32_t row_len, uint32_t bl, uint32_t W, uint32_t D, uint32_t depth,
+                uint64_t row0, uint64_t rows) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= rows * W * NC) return;
+  const uint32_t c = (uint32_t)(idx % NC);
+  const uint32_t w = (uint32_t)((idx / NC) % W);
+  const uint64_t row = idx / ((uint64_t)NC * W);
+  const uint64_t* x = evals + (row0 + row) * row_len + octrev_d(c, depth);
+  uint64_t xs[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) {
+    const uint32_t k = 16 * w + i;
+    xs[i] = k < bl ? x[(uint64_t)NC * k] : 0ull;
+  }
+  uint32_t* dst = (uint32_t*)(planes + ((row * NC + c) * W + w) * D);
+  for (uint32_t p = 0; p < D; p += 2) {
+    uint32_t v0 = 0, v1 = 0;
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      v0 |= (uint32_t)((xs[i] >> p) & 1ull) << i;
+      v1 |= (uint32_t)((xs[i] >> (p + 1)) & 1ull) << i;
+    }
+    dst[p / 2] = v0 | (v1 << 16);
+  }
+}
+
+template <int NC, int S>
+__global__ void __launch_bounds__(256, 2)
+iprs_mfma_kernel(const uint16_t* __restrict__ planes, const v4i* __restrict__ afrag,
+                 const double* __restrict__ tw0, const double* __restrict__ tw1,
+                 int64_t* __restrict__ out, uint32_t bd, uint32_t D, uint32_t nrb, uint32_t npg,
+                 uint64_t row0) {
+  // Bs holds the base sums [c][r_local][p] until stage 0 has read them; the
+  // same bytes then hold this block's stage-1 twiddles [c_lo][r_local][56].
+  constexpr uint32_t kBsWords = NC * 16 * 16;
+  constexpr uint32_t kTw1Doubles = NC == 64 ? 8 * 16 * 56 : 0;
+  static_assert(kTw1Doubles * 2 <= kBsWords, "stage-1 twiddles must fit in the base area");
+  __shared__ int32_t Bs[kBsWords];
+  __shared__ double T0[16 * 56];
+  constexpr uint32_t W = 4 * S;
+  constexpr uint32_t kTiles = NC / 4;  // chunks per wave
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t g = lane >> 4, pcol = lane & 15;
+
+  // Block -> (row, r-block, plane group).  With nrb a multiple of 8, each
+  // XCD (blockIdx mod 8) keeps a fixed 1/8 of the r-blocks, so its L2 holds
+  // only that part of the matrix slices and twiddles.
+  uint32_t rb, pg;
+  uint64_t row;
+  {
+    const uint64_t b = blockIdx.x;
+    if ((nrb & 7) == 0) {
+      const uint32_t xcd = (uint32_t)(b & 7), per = nrb >> 3;
+      const uint64_t q = b >> 3;
+      rb = xcd + 8 * (uint32_t)(q % per);
+      const uint64_t rest = q / per;
+      pg = (uint32_t)(rest % npg);
+      row = rest / npg;
+    } else {
+      rb = (uint32_t)(b % nrb);
+      const uint64_t rest = b / nrb;
+      pg = (uint32_t)(rest % npg);
+      row = rest / npg;
+    }
+  }
+
+  // Issue every global load of phase 1 up front: the plane words of this
+  // wave's chunks, the matrix slices, and the block's stage-0 twiddles.
+  const uint16_t* prow = planes + row * NC * W * D + 16 * pg + pcol;
+  uint32_t pw[kTiles][S];
+#pragma unroll
+  for (uint32_t i = 0; i < kTiles; i++)
+#pragma unroll
+    for (int s = 0; s < S; s++)
+      pw[i][s] = prow[((uint64_t)(wave + 4 * i) * W + 4 * s + g) * D];
+  v4i A[S][3];
+#pragma unroll
+  for (int s = 0; s < S; s++)
+#pragma unroll
+    for (int q = 0; q < 3; q++) A[s][q] = afrag[(((uint64_t)rb * S + s) * 3 + q) * 64 + lane];
+  for (uint32_t i = tid; i < 16 * 56; i += 256) T0[i] = tw0[(uint64_t)rb * 16 * 56 + i];
+
+  // Phase 1: base layer on the matrix cores.
+#pragma unroll
+  for (uint32_t i = 0; i < kTiles; i++) {
+    const uint32_t ci = wave + 4 * i;
+    v4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < S; s++) {
+      const uint32_t w = pw[i][s];
+      v4i bf;
+      bf[0] = spread4(w & 15u);
+      bf[1] = spread4((w >> 4) & 15u);
+      bf[2] = spread4((w >> 8) & 15u);
+      bf[3] = spread4((w >> 12) & 15u);
+      acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][0], bf, acc0, 0, 0, 0);
+      acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][1], bf, acc1, 0, 0, 0);
+      acc2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][2], bf, acc2, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++)
+      Bs[(ci * 16 + 4 * g + v) * 16 + pcol] = acc0[v] + 128 * acc1[v] + 16384 * acc2[v];
+  }
+  __syncthreads();
+
+  // Phase 2: every radix-8 stage of one (r, p) column per thread.
+  const uint32_t rl = tid >> 4, p = tid & 15;
+  const uint32_t r = rb * 16 + rl;
+  const uint64_t N = (uint64_t)bd * NC;
+  int64_t* o = out + ((row0 + row) * N + r) * D + 16 * pg + p;
+  const uint64_t cstride = (uint64_t)bd * D;  // output stride between chunks
+  const double* t0 = T0 + rl * 56;
+
+  double y0[NC];
+#pragma unroll
+  for (int j = 0; j < NC / 8; j++) {
+    double b[8];
+#pragma unroll
+    for (int jj = 0; jj < 8; jj++) b[jj] = (double)Bs[((8 * j + jj) * 16 + rl) * 16 + p];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      double a = b[0];
+#pragma unroll
+      for (int jj = 1; jj < 8; jj++) a = __builtin_fma(b[jj], t0[(jj - 1) * 8 + ((t * jj) & 7)], a);
+      y0[8 * j + t] = a;
+    }
+  }
+
+  if constexpr (NC == 8) {
+#pragma unroll
+    for (int c = 0; c < 8; c++) o[c * cstride] = dbl_to_i64(y0[c]);
+  } else {
+    // Stage-1 twiddles of rows r_local < 16 and c_lo < 8 into the base area.
+    __syncthreads();
+    double* T1 = (double*)Bs;
+    for (uint32_t i = tid; i < kTw1Doubles; i += 256) {
+      const uint32_t cl = i / (16 * 56), rem = i % (16 * 56);
+      T1[i] = tw1[((uint64_t)cl * bd + rb * 16) * 56 + rem];
+    }
+    __syncthreads();
+    const double two21 = 2097152.0, inv21 = 1.0 / 2097152.0;
+#pragma unroll
+    for (int cl = 0; cl < 8; cl++) {
+      double yh[8], yl[8];
+#pragma unroll
+      for (int j = 0; j < 8; j++) {
+        const double v = y0[8 * j + cl];
+        yh[j] = __builtin_floor(v * inv21);
+        yl[j] = __builtin_fma(yh[j], -two21, v);
+      }
+      const double* t1 = T1 + (cl * 16 + rl) * 56;
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        double sh = yh[0], sl = yl[0];
+#pragma unroll
+        for (int j = 1; j < 8; j++) {
+          const double tv = t1[(j - 1) * 8 + ((t * j) & 7)];
+          sh = __builtin_fma(yh[j], tv, sh);
+          sl = __builtin_fma(yl[j], tv, sl);
+        }
+        o[(uint64_t)(8 * t + cl) * cstride] =
+            (int64_t)((uint64_t)dbl_to_i64(sh) << 21) + dbl_to_i64(sl);
+      }
+    }
+  }
+}
+
+template <int NC, int S>
+int launch_main(const zp_code* cd, const uint16_t* planes, uint64_t rows, uint32_t D,
+                int64_t* out, hipStream_t s) {
+  const uint32_t nrb = cd->bd / 16, npg = D / 16;
+  const uint64_t per_row = (uint64_t)nrb * npg;
+  const uint64_t max_rows = std::max<uint64_t>(1, 0x7fffffffull / per_row);
+  for (uint64_t r0 = 0; r0 < rows; r0 += max_rows) {
+    const uint64_t nr = std::min(max_rows, rows - r0);
+    hipLaunchKernelGGL((iprs_mfma_kernel<NC, S>), dim3((uint32_t)(nr * per_row)), dim3(256), 0, s,
+                       planes + r0 * NC * 4 * S * D, (const v4i*)cd->d_afrag, cd->d_tw0f,
+                       cd->d_tw1f, out, cd->bd, D, nrb, npg, r0);
+    ZP_HIP(hipGetLastError());
+  }
+  return ZP_OK;
+}
+
+template <int NC>
+int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D, int64_t* out,
+              hipStream_t s) {
+  const uint32_t S = cd->mfma_ksteps, W = 4 * S;
+  const uint64_t bytes = rows * NC * W * D * sizeof(uint16_t);
+  uint16_t* planes = nullptr;
+  ZP_TRY(dev_alloc((void**)&planes, bytes, s));
+  const uint64_t threads = rows * W * NC;
+  hipLaunchKernelGGL((bitplane_kernel<NC>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,
+                     s, evals, planes, (uint32_t)cd->row_len, cd->bl, W, D, cd->depth, 0ull,
+                     rows);
+  int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "bitplane launch");
+  if (rc == ZP_OK) {
+    switch (S) {
+      case 1: rc = launch_main<NC, 1>(cd, planes, rows, D, out, s); break;
+      case 2: rc = launch_main<NC, 2>(cd, planes, rows, D, out, s); break;
+      case 3: rc = launch_main<NC, 3>(cd, planes, rows, D, out, s); break;
+      case 4: rc = launch_main<NC, 4>(cd, planes, rows, D, out, s); break;
+      default: rc = fail(ZP_ERR_INVALID_PARAM, "unsupported base length for the MFMA encoder");
+    }
+  }
+  dev_free(planes, s);
+  return rc;
+}
+
+}  // namespace
+
+bool iprs_mfma_eligible(const zp_code* cd, uint32_t D) {
+  return cd->d_afrag != nullptr && (D % 16) == 0 && (cd->depth == 1 || cd->depth == 2) &&
+         cd->row_len < (1ull << 31);
+}
+
+int launch_iprs_binary_mfma(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
+                            int64_t* out, hipStream_t s) {
+  if (rows == 0) return ZP_OK;
+  if (cd->depth == 1) return launch_nc<8>(cd, evals, rows, D, out, s);
+  return launch_nc<64>(cd, evals, rows, D, out, s);
+}
+
+}  // namespace zpi

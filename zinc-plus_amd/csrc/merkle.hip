@@ -78,89 +78,160 @@ __device__ __forceinline__ void parent_cv(const uint32_t l[8], uint32_t r_cv[8],
   for (int i = 0; i < 8; i++) r_cv[i] = cv[i];
 }
 
-constexpr int kMaxStack = 40;
-
-// One thread per column j: BLAKE3 of the column's byte stream, in which row
-// r contributes `elem` bytes from rows + (r*width + j)*elem.
-__global__ void __launch_bounds__(256)
-leaf_hash_kernel(const uint8_t* __restrict__ rows, uint64_t nrows, uint64_t width, uint32_t elem,
-                 uint8_t* __restrict__ leaves) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= width) return;
-  const uint64_t total = nrows * elem;
-  const uint64_t nchunks = total == 0 ? 1 : (total + 1023) / 1024;
-  const uint64_t stride = width * elem;
-  const uint8_t* base = rows + j * elem;
-  uint64_t cur_row = 0;
-  uint32_t cur_off = 0;
-  uint32_t stack[kMaxStack][8];
-  int sp = 0;
-  uint32_t cv[8];
-  for (uint64_t ci = 0; ci < nchunks; ci++) {
-    const uint64_t clen = min((uint64_t)1024, total - ci * 1024);
-    const uint32_t nblocks = clen == 0 ? 1 : (uint32_t)((clen + 63) / 64);
+__device__ __forceinline__ void load_block_words(const uint8_t* p, uint32_t m[16]) {
+  const uint4* q = (const uint4*)p;
 #pragma unroll
-    for (int i = 0; i < 8; i++) cv[i] = kIV[i];
-    for (uint32_t b = 0; b < nblocks; b++) {
-      const uint32_t blen = (uint32_t)min((uint64_t)64, clen - (uint64_t)b * 64);
-      uint32_t m[16];
+  for (int i = 0; i < 4; i++) {
+    const uint4 v = q[i];
+    m[4 * i] = v.x;
+    m[4 * i + 1] = v.y;
+    m[4 * i + 2] = v.z;
+    m[4 * i + 3] = v.w;
+  }
+}
+
+// One thread per (column j, chunk ci).  Column j's byte stream is row r's
+// element j (elem bytes at rows + (r*width + j)*elem) for r = 0, 1, ...
+// Chunk ci is stream bytes [1024 ci, 1024 ci + 1024).  With one chunk per
+// column the chunk is the whole input: its last block carries ROOT and the
+// output is the leaf; otherwise the chunk's chaining value goes to cvs.
+// kAligned: elem is a multiple of 64, so every block is 64 contiguous bytes.
+template <bool kAligned>
+__global__ void __launch_bounds__(256)
+chunk_cv_kernel(const uint8_t* __restrict__ rows, uint64_t nrows, uint64_t width, uint32_t elem,
+                uint64_t nchunks, uint32_t* __restrict__ cvs, uint8_t* __restrict__ leaves) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t j = idx % width, ci = idx / width;
+  if (ci >= nchunks) return;
+  const uint64_t total = nrows * elem;
+  const uint64_t c0 = ci * 1024;
+  const uint64_t clen = min((uint64_t)1024, total - c0);
+  const uint32_t nblocks = clen == 0 ? 1 : (uint32_t)((clen + 63) / 64);
+  const uint64_t rstride = width * elem;
+  const uint8_t* col = rows + j * elem;
+  uint32_t cv[8];
+#pragma unroll
+  for (int i = 0; i < 8; i++) cv[i] = kIV[i];
+  // Stream position of the next word as (row, offset within the element).
+  uint64_t row = c0 / elem;
+  uint32_t off = (uint32_t)(c0 - row * elem);
+  for (uint32_t b = 0; b < nblocks; b++) {
+    const uint32_t blen = (uint32_t)min((uint64_t)64, clen - (uint64_t)b * 64);
+    uint32_t m[16];
+    if (kAligned) {
+      load_block_words(col + row * rstride + off, m);
+      off += 64;
+      if (off == elem) {
+        off = 0;
+        row++;
+      }
+    } else {
 #pragma unroll
       for (int w = 0; w < 16; w++) {
         if ((uint32_t)w * 4 < blen) {
-          m[w] = *(const uint32_t*)(base + cur_row * stride + cur_off);
-          cur_off += 4;
-          if (cur_off == elem) {
-            cur_off = 0;
-            cur_row++;
+          m[w] = *(const uint32_t*)(col + row * rstride + off);
+          off += 4;
+          if (off == elem) {
+            off = 0;
+            row++;
           }
         } else {
           m[w] = 0;
         }
       }
-      uint32_t flags = 0;
-      if (b == 0) flags |= kChunkStart;
-      if (b + 1 == nblocks) flags |= kChunkEnd | (nchunks == 1 ? kRoot : 0u);
-      compress(cv, m, ci, blen, flags);
     }
-    if (ci + 1 < nchunks) {
-      uint64_t tc = ci + 1;
-      while ((tc & 1) == 0) {
-        --sp;
-        parent_cv(stack[sp], cv, false);
-        tc >>= 1;
-      }
-#pragma unroll
-      for (int i = 0; i < 8; i++) stack[sp][i] = cv[i];
-      sp++;
-    } else {
-      while (sp > 0) {
-        --sp;
-        parent_cv(stack[sp], cv, sp == 0);
-      }
-    }
+    uint32_t flags = 0;
+    if (b == 0) flags |= kChunkStart;
+    if (b + 1 == nblocks) flags |= kChunkEnd | (nchunks == 1 ? kRoot : 0u);
+    compress(cv, m, ci, blen, flags);
   }
-  uint32_t* dst = (uint32_t*)(leaves + j * 32);
+  if (nchunks == 1) {
+    uint32_t* dst = (uint32_t*)(leaves + j * 32);
 #pragma unroll
-  for (int i = 0; i < 8; i++) dst[i] = cv[i];
+    for (int i = 0; i < 8; i++) dst[i] = cv[i];
+  } else {
+    uint4* dst = (uint4*)(cvs + (ci * width + j) * 8);
+    dst[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+    dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+  }
 }
 
-// One thread per parent node: cur[m] = merge(prev[2m], prev[2m+1]).
+// One level of a column's chunk tree: pairs (2i, 2i+1) merge, an odd last
+// node moves up unchanged.  Pairing from the left level by level gives
+// BLAKE3's tree (left subtree = largest power of two of chunks below the
+// total).  The level with one node is the root: its merge carries ROOT and
+// it is written to the leaf array.
 __global__ void __launch_bounds__(256)
-merkle_layer_kernel(const uint8_t* __restrict__ prev, uint8_t* __restrict__ cur, uint64_t npar,
-                    int root) {
-  const uint64_t m = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= npar) return;
-  const uint32_t* src = (const uint32_t*)(prev + 64 * m);
+cv_level_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint64_t width,
+                uint64_t n_in, uint8_t* __restrict__ leaves) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t n_out = (n_in + 1) / 2;
+  const uint64_t j = idx % width, ci = idx / width;
+  if (ci >= n_out) return;
   uint32_t l[8], r[8];
+  const uint4* a = (const uint4*)(in + (2 * ci * width + j) * 8);
+  uint4 a0 = a[0], a1 = a[1];
+  r[0] = a0.x; r[1] = a0.y; r[2] = a0.z; r[3] = a0.w;
+  r[4] = a1.x; r[5] = a1.y; r[6] = a1.z; r[7] = a1.w;
+  if (2 * ci + 1 < n_in) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) {
-    l[i] = src[i];
-    r[i] = src[8 + i];
+    for (int i = 0; i < 8; i++) l[i] = r[i];
+    const uint4* bq = (const uint4*)(in + ((2 * ci + 1) * width + j) * 8);
+    uint4 b0 = bq[0], b1 = bq[1];
+    r[0] = b0.x; r[1] = b0.y; r[2] = b0.z; r[3] = b0.w;
+    r[4] = b1.x; r[5] = b1.y; r[6] = b1.z; r[7] = b1.w;
+    parent_cv(l, r, n_out == 1);
   }
-  parent_cv(l, r, root != 0);
-  uint32_t* dst = (uint32_t*)(cur + 32 * m);
+  if (n_out == 1) {
+    uint32_t* dst = (uint32_t*)(leaves + j * 32);
 #pragma unroll
-  for (int i = 0; i < 8; i++) dst[i] = r[i];
+    for (int i = 0; i < 8; i++) dst[i] = r[i];
+  } else {
+    uint4* dst = (uint4*)(out + (ci * width + j) * 8);
+    dst[0] = make_uint4(r[0], r[1], r[2], r[3]);
+    dst[1] = make_uint4(r[4], r[5], r[6], r[7]);
+  }
+}
+
+// Merkle layers (merkle.rs:172-221) for n = 2^h leaves stored at layers[0..n).
+// Layer L starts at node offset 2n - 2(n >> L).  A workgroup of 256 threads
+// reads 512 nodes of layer L0 and produces up to 9 layers above them in LDS,
+// writing each node to its place; the layer with one node uses the root merge.
+__global__ void __launch_bounds__(256)
+merkle_tree_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32_t levels) {
+  __shared__ uint32_t buf[512 * 8];
+  const uint32_t t = threadIdx.x;
+  const uint64_t cnt0 = n >> L0;
+  const uint64_t base = (uint64_t)blockIdx.x * 512;
+  const uint32_t m = (uint32_t)min((uint64_t)512, cnt0 - base);
+  const uint32_t* src = (const uint32_t*)(layers + 32 * (2 * n - 2 * cnt0));
+  for (uint32_t i = t; i < m * 8; i += 256) buf[i] = src[base * 8 + i];
+  __syncthreads();
+  uint32_t cur = m;
+  for (uint32_t lv = 1; lv <= levels && cur > 1; lv++) {
+    const uint32_t npar = cur / 2;
+    const uint64_t cnt = n >> (L0 + lv);
+    uint32_t l[8], r[8];
+    if (t < npar) {
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        l[i] = buf[(2 * t) * 8 + i];
+        r[i] = buf[(2 * t + 1) * 8 + i];
+      }
+      parent_cv(l, r, cnt == 1);
+    }
+    __syncthreads();
+    if (t < npar) {
+      uint32_t* dst = (uint32_t*)(layers + 32 * (2 * n - 2 * cnt + (base >> lv) + t));
+#pragma unroll
+      for (int i = 0; i < 8; i++) {
+        buf[t * 8 + i] = r[i];
+        dst[i] = r[i];
+      }
+    }
+    __syncthreads();
+    cur = npar;
+  }
 }
 
 }  // namespace
@@ -169,24 +240,55 @@ int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32
                      uint8_t* leaves, hipStream_t s) {
   if (width == 0) return ZP_OK;
   if (elem % 4 != 0) return fail(ZP_ERR_SHAPE, "element size must be a multiple of 4 bytes");
-  const uint64_t blocks = (width + 255) / 256;
-  hipLaunchKernelGGL(leaf_hash_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, rows, nrows,
-                     width, elem, leaves);
-  ZP_HIP(hipGetLastError());
-  return ZP_OK;
+  const uint64_t total = nrows * elem;
+  const uint64_t nchunks = total == 0 ? 1 : (total + 1023) / 1024;
+  uint32_t* cv[2] = {nullptr, nullptr};
+  if (nchunks > 1) {
+    ZP_TRY(dev_alloc((void**)&cv[0], nchunks * width * 32, s));
+    int rc0 = dev_alloc((void**)&cv[1], ((nchunks + 1) / 2) * width * 32, s);
+    if (rc0 != ZP_OK) {
+      dev_free(cv[0], s);
+      return rc0;
+    }
+  }
+  const uint64_t threads = nchunks * width;
+  const bool aligned = elem % 64 == 0;
+  if (aligned)
+    hipLaunchKernelGGL((chunk_cv_kernel<true>), dim3((uint32_t)((threads + 255) / 256)), dim3(256),
+                       0, s, rows, nrows, width, elem, nchunks, cv[0], leaves);
+  else
+    hipLaunchKernelGGL((chunk_cv_kernel<false>), dim3((uint32_t)((threads + 255) / 256)),
+                       dim3(256), 0, s, rows, nrows, width, elem, nchunks, cv[0], leaves);
+  int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "chunk_cv launch");
+  uint64_t n = nchunks;
+  int cur = 0;
+  while (rc == ZP_OK && n > 1) {
+    const uint64_t n_out = (n + 1) / 2;
+    const uint64_t th = n_out * width;
+    hipLaunchKernelGGL(cv_level_kernel, dim3((uint32_t)((th + 255) / 256)), dim3(256), 0, s,
+                       cv[cur], cv[cur ^ 1], width, n, leaves);
+    if (hipGetLastError() != hipSuccess) rc = fail(ZP_ERR_DEVICE, "cv_level launch");
+    n = n_out;
+    cur ^= 1;
+  }
+  dev_free(cv[0], s);
+  dev_free(cv[1], s);
+  return rc;
 }
 
-// layers: (2n-1) x 32 bytes with the n leaves first (already written).
+// layers: (2n-1) x 32 bytes with the n leaves first (already written); n is
+// a power of two.
 int launch_merkle_layers(uint8_t* layers, uint64_t n, hipStream_t s) {
-  uint64_t off = 0, cnt = n;
-  while (cnt > 1) {
-    const uint64_t npar = cnt / 2;
-    const uint64_t blocks = (npar + 255) / 256;
-    hipLaunchKernelGGL(merkle_layer_kernel, dim3((uint32_t)blocks), dim3(256), 0, s,
-                       layers + 32 * off, layers + 32 * (off + cnt), npar, cnt == 2 ? 1 : 0);
+  uint32_t L = 0;
+  while ((n >> L) > 1) {
+    const uint64_t cnt = n >> L;
+    const uint32_t lg = 63 - (uint32_t)__builtin_clzll(cnt);
+    const uint32_t levels = std::min<uint32_t>(9, lg);
+    const uint64_t blocks = (cnt + 511) / 512;
+    hipLaunchKernelGGL(merkle_tree_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, layers, n, L,
+                       levels);
     ZP_HIP(hipGetLastError());
-    off += cnt;
-    cnt = npar;
+    L += levels;
   }
   return ZP_OK;
 }

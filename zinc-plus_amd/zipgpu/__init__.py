@@ -31,7 +31,7 @@ __all__ = [
     "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory",
     "ZipTypes", "BinPolyTypes", "IntTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
-    "merkle_root", "lib",
+    "merkle_root", "lib", "set_option",
 ]
 
 
@@ -66,6 +66,11 @@ _ERRORS = {L.ZP_ERR_INVALID_PARAM: InvalidPcsParam, L.ZP_ERR_SHAPE: ShapeError,
 
 def lib():
     return L.load()
+
+
+def set_option(name, value):
+    """zp_set_option: "iprs_encoder" 0 = matrix-core encoder (default), 1 = VALU."""
+    _check(L.load().zp_set_option(name.encode(), int(value)))
 
 
 def _check(rc):

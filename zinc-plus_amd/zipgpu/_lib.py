@@ -35,6 +35,7 @@ SIGNATURES = {
     "zp_init": (_int, [_int]),
     "zp_status_string": (ctypes.c_char_p, [_int]),
     "zp_last_error": (ctypes.c_char_p, []),
+    "zp_set_option": (_int, [ctypes.c_char_p, ctypes.c_int64]),
     "zp_code_create": (_int, [_int, _u32, _u64, _i32, ctypes.POINTER(_c_void_p)]),
     "zp_code_destroy": (None, [_c_void_p]),
     "zp_code_row_len": (_u64, [_c_void_p]),
