@@ -64,6 +64,8 @@ const char* zp_last_error(void);
 /* Library options (no reference counterpart):
  *   "iprs_encoder": 0 = matrix-core encoder where eligible (default),
  *                   1 = VALU encoder for every shape.
+ *   "commit_overlap": 1 = hash finished row groups on a second stream while
+ *                   later rows encode (default), 0 = encode, then hash.
  * Returns ZP_ERR_INVALID_PARAM for an unknown name. */
 int zp_set_option(const char* name, int64_t value);
 

@@ -211,6 +211,27 @@ def test_commit_zip_plus_cfg2_full(zg, dev, oracle):
     assert hint.codewords(0).tobytes() == ref["cw"].tobytes()
 
 
+@pytest.mark.parametrize("kind,n,row_len,D", [
+    ("iprs", 1 << 18, 1 << 14, 32), ("iprs", 1 << 16, 1 << 12, 64), ("raa", 1 << 16, 256, 32)])
+def test_commit_overlap_matches_sequential(zg, dev, oracle, kind, n, row_len, D):
+    # The two-stream commit (hash finished row groups while later rows
+    # encode) and the sequential one give the same root as the oracle.
+    x = synth.binary_polys(700 + n, n, D)
+    code = (zg.IprsCode.new_with_optimal_depth(row_len) if kind == "iprs"
+            else zg.RaaCode(row_len))
+    zp = zg.ZipPlus(zg.BinPolyTypes(D))
+    pp = zp.setup(n, code)
+    xd = to_dev(x, dev)
+    _, on = zp.commit(pp, [xd])
+    zg.set_option("commit_overlap", 0)
+    try:
+        _, off = zp.commit(pp, [xd])
+    finally:
+        zg.set_option("commit_overlap", 1)
+    ref = oracle.commit(kind, x.view(np.uint64), pp.num_rows, row_len, D=D)
+    assert on.root == off.root == ref["root"]
+
+
 def test_hint_openings_verify(zg, dev, oracle):
     n, row_len, D = 1 << 12, 256, 32
     x = synth.binary_polys(77, 2 * n, D)

@@ -93,6 +93,24 @@ void dev_release_cache() {
   release_device_locked(dev);
 }
 
+// One auxiliary non-blocking stream per device (default priority), used to
+// run a commit's leaf hashing beside its encoding.
+int aux_stream(hipStream_t* out) {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  int dev = 0;
+  ZP_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = streams.find(dev);
+  if (it == streams.end()) {
+    hipStream_t st;
+    ZP_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    it = streams.emplace(dev, st).first;
+  }
+  *out = it->second;
+  return ZP_OK;
+}
+
 // A per-thread pinned staging buffer for small device-to-host results.
 uint8_t* pinned_scratch(size_t bytes) {
   static thread_local uint8_t* buf = nullptr;

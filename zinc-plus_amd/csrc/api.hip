@@ -34,6 +34,8 @@ bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
 // Selects the VALU IPRS encoder instead of the matrix-core one (set by
 // zp_set_option; used to cross-check the two device paths).
 int g_force_valu = 0;
+// Disables the two-stream overlap of encoding and hashing inside a commit.
+int g_no_overlap = 0;
 
 template <class T>
 int upload(const std::vector<T>& host, T** dev) {
@@ -96,6 +98,58 @@ int encode(const zp_code* c, int eval_type, uint32_t D, const void* evals, uint6
 
 uint64_t eval_bytes(int eval_type) { return eval_type == ZP_EVAL_I32 ? 4 : 8; }
 
+// Encode all rows of all polys, then hash the leaves.  For a single poly
+// whose rows split into whole 1 KiB BLAKE3 chunks, the rows are encoded in
+// groups on `s` while the chunks of finished groups are hashed on the
+// auxiliary stream; the streams join before the chunk trees are merged.
+int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
+                           uint64_t num_rows, uint32_t batch, uint64_t in_stride, zp_hint* h,
+                           hipStream_t s) {
+  const uint64_t N = h->N, elem = h->elem, rows_all = num_rows * batch;
+  const uint64_t rpc = (elem <= 1024 && 1024 % elem == 0) ? 1024 / elem : 0;  // rows per chunk
+  const uint64_t nchunks = leaf_chunks(rows_all, (uint32_t)elem);
+  const uint32_t kGroups = 4;
+  const bool pipelined = batch == 1 && rpc && num_rows % (rpc * kGroups) == 0 &&
+                         nchunks >= kGroups && !g_no_overlap;
+  if (!pipelined) {
+    for (uint32_t b = 0; b < batch; b++)
+      ZP_TRY(encode(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride, num_rows,
+                    h->d_cw + b * h->per_poly, s));
+    return launch_leaf_hash(h->d_cw, rows_all, N, (uint32_t)elem, h->d_layers, s);
+  }
+  hipStream_t s2;
+  ZP_TRY(aux_stream(&s2));
+  uint32_t* cv[2] = {nullptr, nullptr};
+  ZP_TRY(dev_alloc((void**)&cv[0], nchunks * N * 32, s));
+  int rc = dev_alloc((void**)&cv[1], ((nchunks + 1) / 2) * N * 32, s);
+  hipEvent_t ev[kGroups + 1] = {};
+  for (uint32_t q = 0; rc == ZP_OK && q <= kGroups; q++)
+    if (hipEventCreateWithFlags(&ev[q], hipEventDisableTiming) != hipSuccess)
+      rc = fail(ZP_ERR_DEVICE, "event creation failed");
+  const uint64_t rows_q = num_rows / kGroups, chunks_q = nchunks / kGroups;
+  for (uint32_t q = 0; rc == ZP_OK && q < kGroups; q++) {
+    rc = encode(code, eval_type, D,
+                (const uint8_t*)d_evals + q * rows_q * code->row_len * eval_bytes(eval_type),
+                rows_q, h->d_cw + q * rows_q * N * elem, s);
+    if (rc == ZP_OK && (hipEventRecord(ev[q], s) != hipSuccess ||
+                        hipStreamWaitEvent(s2, ev[q], 0) != hipSuccess))
+      rc = fail(ZP_ERR_DEVICE, "event record/wait failed");
+    if (rc == ZP_OK)
+      rc = launch_chunk_cvs(h->d_cw, rows_all, N, (uint32_t)elem, q * chunks_q,
+                            (q + 1) * chunks_q, cv[0], h->d_layers, s2);
+  }
+  // Join: later work on `s` (and the reuse of cv on `s`) waits for s2.
+  if (hipEventRecord(ev[kGroups], s2) != hipSuccess ||
+      hipStreamWaitEvent(s, ev[kGroups], 0) != hipSuccess)
+    rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "stream join failed") : rc;
+  if (rc == ZP_OK) rc = launch_cv_merge(cv[0], cv[1], nchunks, N, h->d_layers, s);
+  for (auto e : ev)
+    if (e) (void)hipEventDestroy(e);
+  dev_free(cv[0], s);
+  dev_free(cv[1], s);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -124,6 +178,11 @@ int zp_set_option(const char* name, int64_t value) {
   if (std::strcmp(name, "iprs_encoder") == 0) {
     if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "iprs_encoder is 0 or 1");
     g_force_valu = (int)value;
+    return ZP_OK;
+  }
+  if (std::strcmp(name, "commit_overlap") == 0) {
+    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "commit_overlap is 0 or 1");
+    g_no_overlap = value ? 0 : 1;
     return ZP_OK;
   }
   return fail(ZP_ERR_INVALID_PARAM, std::string("unknown option ") + name);
@@ -262,11 +321,8 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_eval
   uint8_t* pin = pinned_scratch(32);
   if (rc == ZP_OK && !pin) rc = fail(ZP_ERR_OOM, "pinned staging allocation failed");
   const uint64_t in_stride = num_rows * code->row_len * eval_bytes(eval_type);
-  for (uint32_t b = 0; rc == ZP_OK && b < batch; b++)
-    rc = encode(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride, num_rows,
-                h->d_cw + b * h->per_poly, s);
-  if (rc == ZP_OK) rc = launch_leaf_hash(h->d_cw, num_rows * batch, N, (uint32_t)elem,
-                                         h->d_layers, s);
+  if (rc == ZP_OK) rc = commit_encode_and_hash(code, eval_type, D, d_evals, num_rows, batch,
+                                               in_stride, h, s);
   if (rc == ZP_OK) rc = launch_merkle_layers(h->d_layers, N, s);
   if (rc == ZP_OK && hipMemcpyAsync(pin, h->d_layers + (2 * N - 2) * 32, 32,
                                     hipMemcpyDeviceToHost, s) != hipSuccess)

@@ -82,6 +82,14 @@ int launch_raa_binary(const zp_code* c, const uint64_t* evals, uint64_t rows, ui
                       int32_t* out, hipStream_t s);
 int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
                      uint8_t* leaves, hipStream_t s);
+uint64_t leaf_chunks(uint64_t nrows, uint32_t elem);
+int launch_chunk_cvs(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
+                     uint64_t ci_begin, uint64_t ci_end, uint32_t* cvs, uint8_t* leaves,
+                     hipStream_t s);
+int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
+                    uint8_t* leaves, hipStream_t s);
+// A second stream (with events) per device for overlapping commit stages.
+int aux_stream(hipStream_t* out);
 int launch_merkle_layers(uint8_t* layers, uint64_t n, hipStream_t s);
 int launch_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
                         uint32_t batch, const uint64_t* evals, const int64_t* alphas,

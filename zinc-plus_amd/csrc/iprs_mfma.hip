@@ -18,8 +18,8 @@
 // results equal the reference's wrapping-i64 arithmetic bit for bit because
 // no value on this path exceeds 2^59.
 //
-// Work split: a pre-pass writes, per row and chunk c, the 16-bit words
-// P[c][w][p] = bits of plane p for k = 16w .. 16w+15.  The main kernel owns
+// Work split: a pre-pass writes, per row and chunk c, the words P[c][w][p]
+// holding the bits of plane p for k = 16w .. 16w+15.  The main kernel owns
 // (row, 16 base rows r, 16 planes p) and all 8^d chunks: MFMA tiles are
 // (16 r) x (16 planes) for one chunk, the summed base values go to LDS, and
 // each thread then runs every stage of one (r, p) column and writes its
@@ -46,15 +46,17 @@ __device__ __forceinline__ int64_t dbl_to_i64(double x) {
   return __double_as_longlong(x + magic) - __double_as_longlong(magic);
 }
 
-// Bits 0..3 of `nib` -> bytes 0..3 (each 0 or 1).
-__device__ __forceinline__ int spread4(uint32_t nib) {
-  return (int)((nib * 0x00204081u) & 0x01010101u);
-}
+constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+constexpr uint64_t kMagicBits = 0x4338000000000000ull;
+constexpr uint64_t kMagicFix = (kMagicBits << 21) + kMagicBits;
 
-// planes[(row*NC + c)*W + w][p] = sum_i bit_p(x[octrev(c) + NC*(16w+i)]) << i
+// planes[((row*NC + c)*W + w)*D + p] holds the 16 bits of plane p for
+// k = 16w .. 16w+15 in "spread" order: bit i = 4v + b of the group sits at
+// position 8b + v, so (word >> v) & 0x01010101 is operand register v of the
+// MFMA B fragment (bytes b = 0..3 for k = 16w + 4v + b) in two instructions.
 template <int NC>
 __global__ void __launch_bounds__(256)
-bitplane_kernel(const uint64_t* __restrict__ evals, uint16_t* __restrict__ planes,
+bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ planes,
                 uint32_t row_len, uint32_t bl, uint32_t W, uint32_t D, uint32_t depth,
                 uint64_t row0, uint64_t rows) {
   const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -69,21 +71,18 @@ bitplane_kernel(const uint64_t* __restrict__ evals, uint16_t* __restrict__ plane
     const uint32_t k = 16 * w + i;
     xs[i] = k < bl ? x[(uint64_t)NC * k] : 0ull;
   }
-  uint32_t* dst = (uint32_t*)(planes + ((row * NC + c) * W + w) * D);
-  for (uint32_t p = 0; p < D; p += 2) {
-    uint32_t v0 = 0, v1 = 0;
+  uint32_t* dst = planes + ((row * NC + c) * W + w) * D;
+  for (uint32_t p = 0; p < D; p++) {
+    uint32_t v = 0;
 #pragma unroll
-    for (int i = 0; i < 16; i++) {
-      v0 |= (uint32_t)((xs[i] >> p) & 1ull) << i;
-      v1 |= (uint32_t)((xs[i] >> (p + 1)) & 1ull) << i;
-    }
-    dst[p / 2] = v0 | (v1 << 16);
+    for (int i = 0; i < 16; i++) v |= (uint32_t)((xs[i] >> p) & 1ull) << (8 * (i & 3) + (i >> 2));
+    dst[p] = v;
   }
 }
 
 template <int NC, int S>
 __global__ void __launch_bounds__(256, 2)
-iprs_mfma_kernel(const uint16_t* __restrict__ planes, const v4i* __restrict__ afrag,
+iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ afrag,
                  const double* __restrict__ tw0, const double* __restrict__ tw1,
                  int64_t* __restrict__ out, uint32_t bd, uint32_t D, uint32_t nrb, uint32_t npg,
                  uint64_t row0) {
@@ -123,7 +122,7 @@ iprs_mfma_kernel(const uint16_t* __restrict__ planes, const v4i* __restrict__ af
 
   // Issue every global load of phase 1 up front: the plane words of this
   // wave's chunks, the matrix slices, and the block's stage-0 twiddles.
-  const uint16_t* prow = planes + row * NC * W * D + 16 * pg + pcol;
+  const uint32_t* prow = planes + row * NC * W * D + 16 * pg + pcol;
   uint32_t pw[kTiles][S];
 #pragma unroll
   for (uint32_t i = 0; i < kTiles; i++)
@@ -146,10 +145,10 @@ iprs_mfma_kernel(const uint16_t* __restrict__ planes, const v4i* __restrict__ af
     for (int s = 0; s < S; s++) {
       const uint32_t w = pw[i][s];
       v4i bf;
-      bf[0] = spread4(w & 15u);
-      bf[1] = spread4((w >> 4) & 15u);
-      bf[2] = spread4((w >> 8) & 15u);
-      bf[3] = spread4((w >> 12) & 15u);
+      bf[0] = (int)(w & 0x01010101u);
+      bf[1] = (int)((w >> 1) & 0x01010101u);
+      bf[2] = (int)((w >> 2) & 0x01010101u);
+      bf[3] = (int)((w >> 3) & 0x01010101u);
       acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][0], bf, acc0, 0, 0, 0);
       acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][1], bf, acc1, 0, 0, 0);
       acc2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][2], bf, acc2, 0, 0, 0);
@@ -164,8 +163,11 @@ iprs_mfma_kernel(const uint16_t* __restrict__ planes, const v4i* __restrict__ af
   const uint32_t rl = tid >> 4, p = tid & 15;
   const uint32_t r = rb * 16 + rl;
   const uint64_t N = (uint64_t)bd * NC;
-  int64_t* o = out + ((row0 + row) * N + r) * D + 16 * pg + p;
-  const uint64_t cstride = (uint64_t)bd * D;  // output stride between chunks
+  // One row's codewords span N*D*8 <= 32 MiB, so 32-bit element offsets from
+  // the row base address every output.
+  char* orow = (char*)(out + (row0 + row) * N * D);
+  const uint32_t obase = (r * D + 16 * pg + p) * 8;  // byte offset of chunk 0
+  const uint32_t cstride = bd * D * 8;               // byte stride between chunks
   const double* t0 = T0 + rl * 56;
 
   double y0[NC];
@@ -174,18 +176,20 @@ iprs_mfma_kernel(const uint16_t* __restrict__ planes, const v4i* __restrict__ af
     double b[8];
 #pragma unroll
     for (int jj = 0; jj < 8; jj++) b[jj] = (double)Bs[((8 * j + jj) * 16 + rl) * 16 + p];
+    // Eight independent accumulation chains (one per output t).
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-      double a = b[0];
+    for (int t = 0; t < 8; t++) y0[8 * j + t] = b[0];
 #pragma unroll
-      for (int jj = 1; jj < 8; jj++) a = __builtin_fma(b[jj], t0[(jj - 1) * 8 + ((t * jj) & 7)], a);
-      y0[8 * j + t] = a;
-    }
+    for (int jj = 1; jj < 8; jj++)
+#pragma unroll
+      for (int t = 0; t < 8; t++)
+        y0[8 * j + t] = __builtin_fma(b[jj], t0[(jj - 1) * 8 + ((t * jj) & 7)], y0[8 * j + t]);
   }
 
   if constexpr (NC == 8) {
 #pragma unroll
-    for (int c = 0; c < 8; c++) o[c * cstride] = dbl_to_i64(y0[c]);
+    for (int c = 0; c < 8; c++)
+      *(int64_t*)(orow + (obase + (uint32_t)c * cstride)) = dbl_to_i64(y0[c]);
   } else {
     // Stage-1 twiddles of rows r_local < 16 and c_lo < 8 into the base area.
     __syncthreads();
@@ -206,24 +210,37 @@ iprs_mfma_kernel(const uint16_t* __restrict__ planes, const v4i* __restrict__ af
         yl[j] = __builtin_fma(yh[j], -two21, v);
       }
       const double* t1 = T1 + (cl * 16 + rl) * 56;
+      // Sixteen independent chains: high and low halves of the 8 outputs.
+      double sh[8], sl[8];
 #pragma unroll
       for (int t = 0; t < 8; t++) {
-        double sh = yh[0], sl = yl[0];
+        sh[t] = yh[0];
+        sl[t] = yl[0];
+      }
 #pragma unroll
-        for (int j = 1; j < 8; j++) {
+      for (int j = 1; j < 8; j++)
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
           const double tv = t1[(j - 1) * 8 + ((t * j) & 7)];
-          sh = __builtin_fma(yh[j], tv, sh);
-          sl = __builtin_fma(yl[j], tv, sl);
+          sh[t] = __builtin_fma(yh[j], tv, sh[t]);
+          sl[t] = __builtin_fma(yl[j], tv, sl[t]);
         }
-        o[(uint64_t)(8 * t + cl) * cstride] =
-            (int64_t)((uint64_t)dbl_to_i64(sh) << 21) + dbl_to_i64(sl);
+      uint32_t ooff = obase + (uint32_t)cl * cstride;  // output chunk 8t + cl
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        // (sh + M) and (sl + M) hold sh and sl in their low mantissa bits;
+        // out = (bits(sh+M) << 21) + bits(sl+M) - bits(M) * (2^21 + 1).
+        const uint64_t hb = (uint64_t)__double_as_longlong(sh[t] + kMagic);
+        const uint64_t lb = (uint64_t)__double_as_longlong(sl[t] + kMagic);
+        *(int64_t*)(orow + ooff) = (int64_t)((hb << 21) + lb - kMagicFix);
+        ooff += 8 * cstride;
       }
     }
   }
 }
 
 template <int NC, int S>
-int launch_main(const zp_code* cd, const uint16_t* planes, uint64_t rows, uint32_t D,
+int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32_t D,
                 int64_t* out, hipStream_t s) {
   const uint32_t nrb = cd->bd / 16, npg = D / 16;
   const uint64_t per_row = (uint64_t)nrb * npg;
@@ -242,8 +259,8 @@ template <int NC>
 int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D, int64_t* out,
               hipStream_t s) {
   const uint32_t S = cd->mfma_ksteps, W = 4 * S;
-  const uint64_t bytes = rows * NC * W * D * sizeof(uint16_t);
-  uint16_t* planes = nullptr;
+  const uint64_t bytes = rows * NC * W * D * sizeof(uint32_t);
+  uint32_t* planes = nullptr;
   ZP_TRY(dev_alloc((void**)&planes, bytes, s));
   const uint64_t threads = rows * W * NC;
   hipLaunchKernelGGL((bitplane_kernel<NC>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,

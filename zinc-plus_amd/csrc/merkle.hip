@@ -99,10 +99,11 @@ __device__ __forceinline__ void load_block_words(const uint8_t* p, uint32_t m[16
 template <bool kAligned>
 __global__ void __launch_bounds__(256)
 chunk_cv_kernel(const uint8_t* __restrict__ rows, uint64_t nrows, uint64_t width, uint32_t elem,
-                uint64_t nchunks, uint32_t* __restrict__ cvs, uint8_t* __restrict__ leaves) {
+                uint64_t nchunks, uint64_t ci_begin, uint64_t ci_end, uint32_t* __restrict__ cvs,
+                uint8_t* __restrict__ leaves) {
   const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t j = idx % width, ci = idx / width;
-  if (ci >= nchunks) return;
+  const uint64_t j = idx % width, ci = ci_begin + idx / width;
+  if (ci >= ci_end) return;
   const uint64_t total = nrows * elem;
   const uint64_t c0 = ci * 1024;
   const uint64_t clen = min((uint64_t)1024, total - c0);
@@ -236,12 +237,54 @@ merkle_tree_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32
 
 }  // namespace
 
+uint64_t leaf_chunks(uint64_t nrows, uint32_t elem) {
+  const uint64_t total = nrows * elem;
+  return total == 0 ? 1 : (total + 1023) / 1024;
+}
+
+// Chunk chaining values for chunks [ci_begin, ci_end) of every column (or the
+// leaves themselves when each column is a single chunk).
+int launch_chunk_cvs(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
+                     uint64_t ci_begin, uint64_t ci_end, uint32_t* cvs, uint8_t* leaves,
+                     hipStream_t s) {
+  if (elem % 4 != 0) return fail(ZP_ERR_SHAPE, "element size must be a multiple of 4 bytes");
+  const uint64_t nchunks = leaf_chunks(nrows, elem);
+  if (ci_end <= ci_begin || width == 0) return ZP_OK;
+  const uint64_t threads = (ci_end - ci_begin) * width;
+  const uint32_t blocks = (uint32_t)((threads + 255) / 256);
+  if (elem % 64 == 0)
+    hipLaunchKernelGGL((chunk_cv_kernel<true>), dim3(blocks), dim3(256), 0, s, rows, nrows, width,
+                       elem, nchunks, ci_begin, ci_end, cvs, leaves);
+  else
+    hipLaunchKernelGGL((chunk_cv_kernel<false>), dim3(blocks), dim3(256), 0, s, rows, nrows,
+                       width, elem, nchunks, ci_begin, ci_end, cvs, leaves);
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
+}
+
+// Merges each column's chunk chaining values (cvs, nchunks x width) up to
+// the leaf; `tmp` holds ceil(nchunks/2) x width values.
+int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
+                    uint8_t* leaves, hipStream_t s) {
+  uint32_t* buf[2] = {cvs, tmp};
+  uint64_t n = nchunks;
+  int cur = 0;
+  while (n > 1) {
+    const uint64_t n_out = (n + 1) / 2;
+    const uint64_t th = n_out * width;
+    hipLaunchKernelGGL(cv_level_kernel, dim3((uint32_t)((th + 255) / 256)), dim3(256), 0, s,
+                       buf[cur], buf[cur ^ 1], width, n, leaves);
+    ZP_HIP(hipGetLastError());
+    n = n_out;
+    cur ^= 1;
+  }
+  return ZP_OK;
+}
+
 int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
                      uint8_t* leaves, hipStream_t s) {
   if (width == 0) return ZP_OK;
-  if (elem % 4 != 0) return fail(ZP_ERR_SHAPE, "element size must be a multiple of 4 bytes");
-  const uint64_t total = nrows * elem;
-  const uint64_t nchunks = total == 0 ? 1 : (total + 1023) / 1024;
+  const uint64_t nchunks = leaf_chunks(nrows, elem);
   uint32_t* cv[2] = {nullptr, nullptr};
   if (nchunks > 1) {
     ZP_TRY(dev_alloc((void**)&cv[0], nchunks * width * 32, s));
@@ -251,26 +294,8 @@ int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32
       return rc0;
     }
   }
-  const uint64_t threads = nchunks * width;
-  const bool aligned = elem % 64 == 0;
-  if (aligned)
-    hipLaunchKernelGGL((chunk_cv_kernel<true>), dim3((uint32_t)((threads + 255) / 256)), dim3(256),
-                       0, s, rows, nrows, width, elem, nchunks, cv[0], leaves);
-  else
-    hipLaunchKernelGGL((chunk_cv_kernel<false>), dim3((uint32_t)((threads + 255) / 256)),
-                       dim3(256), 0, s, rows, nrows, width, elem, nchunks, cv[0], leaves);
-  int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "chunk_cv launch");
-  uint64_t n = nchunks;
-  int cur = 0;
-  while (rc == ZP_OK && n > 1) {
-    const uint64_t n_out = (n + 1) / 2;
-    const uint64_t th = n_out * width;
-    hipLaunchKernelGGL(cv_level_kernel, dim3((uint32_t)((th + 255) / 256)), dim3(256), 0, s,
-                       cv[cur], cv[cur ^ 1], width, n, leaves);
-    if (hipGetLastError() != hipSuccess) rc = fail(ZP_ERR_DEVICE, "cv_level launch");
-    n = n_out;
-    cur ^= 1;
-  }
+  int rc = launch_chunk_cvs(rows, nrows, width, elem, 0, nchunks, cv[0], leaves, s);
+  if (rc == ZP_OK) rc = launch_cv_merge(cv[0], cv[1], nchunks, width, leaves, s);
   dev_free(cv[0], s);
   dev_free(cv[1], s);
   return rc;
