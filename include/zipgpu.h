@@ -64,6 +64,10 @@ const char* zp_last_error(void);
 /* Library options (no reference counterpart):
  *   "iprs_encoder": 0 = matrix-core encoder where eligible (default),
  *                   1 = VALU encoder for every shape.
+ *   "raa_plane_groups": 1 = RAA rows by groups of planes for every shape,
+ *                   0 = the full-row kernel where eligible (default).
+ *   "raa_wide_groups": plane-group kernel at codeword length 8192 with 8
+ *                   planes per workgroup (1, default) or 4 (0).
  *   "commit_overlap": 1 = hash finished row groups on a second stream while
  *                   later rows encode (default), 0 = encode, then hash.
  * Returns ZP_ERR_INVALID_PARAM for an unknown name. */

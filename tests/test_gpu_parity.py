@@ -124,6 +124,27 @@ def test_raa_binary_encode(zg, dev, oracle, row_len, D, rows, flip):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("row_len,D,rows", [(2048, 64, 3), (1024, 32, 2), (256, 64, 2),
+                                             (4096, 32, 1), (512, 32, 4)])
+def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows):
+    # The full-row RAA kernel (compact first scan, all planes per workgroup)
+    # and the plane-group kernel are separate device paths; both must equal
+    # the oracle.
+    say("raa kernels", row_len, D, rows)
+    x = synth.binary_polys(800 + row_len + D, rows * row_len, D)
+    code = zg.RaaCode(row_len)
+    types = zg.BinPolyTypes(D)
+    xd = to_dev(x, dev)
+    full = code.encode_rows(xd, rows, types).cpu().numpy()
+    zg.set_option("raa_plane_groups", 1)
+    try:
+        groups = code.encode_rows(xd, rows, types).cpu().numpy()
+    finally:
+        zg.set_option("raa_plane_groups", 0)
+    want = oracle.raa_encode_binary(x.view(np.uint64), rows, row_len, D)
+    assert np.array_equal(full, want) and np.array_equal(groups, want)
+
+
 def test_raa_width_assert(zg, dev):
     # raa.rs:52-69: 1 + 2*14 + 2*2 = 33 bits do not fit i32 codewords.
     code = zg.RaaCode(1 << 14)

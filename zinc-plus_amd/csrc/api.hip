@@ -180,6 +180,16 @@ int zp_set_option(const char* name, int64_t value) {
     g_force_valu = (int)value;
     return ZP_OK;
   }
+  if (std::strcmp(name, "raa_wide_groups") == 0) {
+    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "raa_wide_groups is 0 or 1");
+    g_raa_wide = (int)value;
+    return ZP_OK;
+  }
+  if (std::strcmp(name, "raa_plane_groups") == 0) {
+    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "raa_plane_groups is 0 or 1");
+    g_raa_planes = (int)value;
+    return ZP_OK;
+  }
   if (std::strcmp(name, "commit_overlap") == 0) {
     if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "commit_overlap is 0 or 1");
     g_no_overlap = value ? 0 : 1;
@@ -416,7 +426,7 @@ int zp_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
   if (L < 3 || L > 8) return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
   if (batch == 0 || num_rows == 0 || row_len == 0) return fail(ZP_ERR_SHAPE, "empty input");
   void* scratch = nullptr;
-  ZP_TRY(dev_alloc(&scratch, combined_row_scratch_bytes(D, row_len, batch), s));
+  ZP_TRY(dev_alloc(&scratch, combined_row_scratch_bytes(D, row_len, batch, num_rows), s));
   int rc = launch_combined_row(D, L, num_rows, row_len, batch, d_evals, d_alphas, d_coeffs, d_out,
                                scratch, s);
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)

@@ -9,48 +9,71 @@
 // The device evaluates the same exact value with the sums reordered:
 //   comb[col] = sum_b sum_c alpha_{b,c} * S_{b,c}[col],
 //   S_{b,c}[col] = sum_j s_j * bit_c(x_b[j*row_len + col]).
-// |S| < 2^(127 + log2 num_rows) fits three signed limbs; every product and
-// the final sum are taken mod 2^(64L) in two's complement, which is the
-// reference's Int<L> result.
+// |S| < 2^(127 + log2 num_rows) fits three signed 64-bit limbs; every
+// product and the final sum are taken mod 2^(64L) in two's complement,
+// which is the reference's Int<L> result.
+//
+// Kernel 1: one thread per (poly b, plane group of 8, row segment, column)
+// sums s_j over its segment for the 8 planes (one eval load per row).
+// Kernel 2: one thread per (column, plane) adds the segment partials,
+// multiplies by alpha, and the block reduces the plane products.
 #include "common.h"
 
 namespace zpi {
 namespace {
 
-// S_{b,c}[col] for every (b, c, col); adjacent threads take adjacent
-// columns so the eval loads coalesce, and s_j is uniform per iteration.
+constexpr uint32_t kSegRows = 128;
+
+__device__ __forceinline__ void add3(uint64_t& s0, uint64_t& s1, uint64_t& s2, uint64_t a0,
+                                     uint64_t a1, uint64_t a2) {
+  const uint64_t t0 = s0 + a0;
+  const uint64_t c0 = t0 < s0;
+  const uint64_t t1a = s1 + a1;
+  const uint64_t c1a = t1a < s1;
+  const uint64_t t1 = t1a + c0;
+  const uint64_t c1 = c1a | (t1 < t1a);
+  s2 = s2 + a2 + c1;
+  s1 = t1;
+  s0 = t0;
+}
+
+// part[((b*nseg + seg)*D + c)*row_len + col][3]
 __global__ void __launch_bounds__(256)
 comb_partial_kernel(const uint64_t* __restrict__ evals, const int64_t* __restrict__ coeffs,
                     uint64_t* __restrict__ part, uint64_t num_rows, uint64_t row_len, uint32_t D,
-                    uint32_t batch) {
+                    uint32_t ngroups, uint32_t nseg, uint32_t batch) {
   const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t total = (uint64_t)batch * D * row_len;
+  const uint64_t total = (uint64_t)batch * ngroups * nseg * row_len;
   if (idx >= total) return;
   const uint64_t col = idx % row_len;
-  const uint32_t c = (uint32_t)((idx / row_len) % D);
-  const uint64_t b = idx / (row_len * D);
+  uint64_t rest = idx / row_len;
+  const uint32_t seg = (uint32_t)(rest % nseg);
+  rest /= nseg;
+  const uint32_t g = (uint32_t)(rest % ngroups);
+  const uint64_t b = rest / ngroups;
+  const uint64_t j0 = (uint64_t)seg * kSegRows;
+  const uint64_t j1 = min(num_rows, j0 + kSegRows);
   const uint64_t* x = evals + b * num_rows * row_len + col;
-  uint64_t s0 = 0, s1 = 0, s2 = 0;
-  for (uint64_t j = 0; j < num_rows; j++) {
-    const uint64_t w = x[j * row_len];
-    if ((w >> c) & 1ull) {
-      const uint64_t lo = (uint64_t)coeffs[2 * j], hi = (uint64_t)coeffs[2 * j + 1];
-      const uint64_t ext = (int64_t)hi < 0 ? ~0ull : 0ull;
-      const uint64_t t0 = s0 + lo;
-      const uint64_t c0 = t0 < s0;
-      const uint64_t t1a = s1 + hi;
-      const uint64_t c1a = t1a < s1;
-      const uint64_t t1 = t1a + c0;
-      const uint64_t c1 = c1a | (t1 < t1a);
-      s2 = s2 + ext + c1;
-      s1 = t1;
-      s0 = t0;
+  uint64_t s[8][3];
+#pragma unroll
+  for (int k = 0; k < 8; k++) s[k][0] = s[k][1] = s[k][2] = 0;
+  for (uint64_t j = j0; j < j1; j++) {
+    const uint32_t bits = (uint32_t)(x[j * row_len] >> (8 * g)) & 0xffu;
+    const uint64_t lo = (uint64_t)coeffs[2 * j], hi = (uint64_t)coeffs[2 * j + 1];
+    const uint64_t ext = (int64_t)hi < 0 ? ~0ull : 0ull;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint64_t m = 0ull - (uint64_t)((bits >> k) & 1u);
+      add3(s[k][0], s[k][1], s[k][2], lo & m, hi & m, ext & m);
     }
   }
-  uint64_t* dst = part + idx * 3;
-  dst[0] = s0;
-  dst[1] = s1;
-  dst[2] = s2;
+  const uint32_t nc = min(8u, D - 8 * g);
+  for (uint32_t k = 0; k < nc; k++) {
+    uint64_t* dst = part + ((((b * nseg + seg) * D) + 8 * g + k) * row_len + col) * 3;
+    dst[0] = s[k][0];
+    dst[1] = s[k][1];
+    dst[2] = s[k][2];
+  }
 }
 
 // acc += a * b mod 2^(64L), with a, b given as sign-extended limb arrays.
@@ -87,23 +110,47 @@ __device__ __forceinline__ void mac_mod(uint64_t acc[L], const uint64_t a[L], co
 }
 
 template <int L>
+__device__ __forceinline__ void add_mod(uint64_t acc[L], const uint64_t a[L]) {
+  uint64_t carry = 0;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    uint64_t s = acc[i] + a[i];
+    uint64_t c1 = s < a[i];
+    s += carry;
+    c1 += s < carry;
+    acc[i] = s;
+    carry = c1;
+  }
+}
+
+// Block: 64 threads per column (planes c, c+64, ... for D > 64 is not
+// needed: D <= 64), kColsPerBlock columns.
+constexpr uint32_t kColsPerBlock = 4;
+
+template <int L>
 __global__ void __launch_bounds__(256)
 comb_final_kernel(const uint64_t* __restrict__ part, const int64_t* __restrict__ alphas,
-                  uint64_t* __restrict__ out, uint64_t row_len, uint32_t D, uint32_t batch) {
-  const uint64_t col = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (col >= row_len) return;
+                  uint64_t* __restrict__ out, uint64_t row_len, uint32_t D, uint32_t nseg,
+                  uint32_t batch) {
+  __shared__ uint64_t red[kColsPerBlock][64][L];
+  const uint32_t c = threadIdx.x & 63, lc = threadIdx.x >> 6;
+  const uint64_t col = (uint64_t)blockIdx.x * kColsPerBlock + lc;
   uint64_t acc[L];
 #pragma unroll
   for (int i = 0; i < L; i++) acc[i] = 0;
-  for (uint32_t b = 0; b < batch; b++)
-    for (uint32_t c = 0; c < D; c++) {
-      const uint64_t* p = part + (((uint64_t)b * D + c) * row_len + col) * 3;
+  if (col < row_len && c < D) {
+    for (uint32_t b = 0; b < batch; b++) {
+      uint64_t s0 = 0, s1 = 0, s2 = 0;
+      for (uint32_t seg = 0; seg < nseg; seg++) {
+        const uint64_t* p = part + ((((uint64_t)b * nseg + seg) * D + c) * row_len + col) * 3;
+        add3(s0, s1, s2, p[0], p[1], p[2]);
+      }
       uint64_t a[L], al[L];
-      a[0] = p[0];
-      a[1] = p[1];
-      const uint64_t ea = (int64_t)p[2] < 0 ? ~0ull : 0ull;
+      a[0] = s0;
+      a[1] = s1;
+      const uint64_t ea = (int64_t)s2 < 0 ? ~0ull : 0ull;
 #pragma unroll
-      for (int i = 2; i < L; i++) a[i] = i == 2 ? p[2] : ea;
+      for (int i = 2; i < L; i++) a[i] = i == 2 ? s2 : ea;
       const int64_t* av = alphas + 2 * ((uint64_t)b * D + c);
       al[0] = (uint64_t)av[0];
       al[1] = (uint64_t)av[1];
@@ -112,40 +159,65 @@ comb_final_kernel(const uint64_t* __restrict__ part, const int64_t* __restrict__
       for (int i = 2; i < L; i++) al[i] = eb;
       mac_mod<L>(acc, a, al);
     }
-  uint64_t* dst = out + col * L;
+  }
 #pragma unroll
-  for (int i = 0; i < L; i++) dst[i] = acc[i];
+  for (int i = 0; i < L; i++) red[lc][c][i] = acc[i];
+  __syncthreads();
+  for (uint32_t off = 32; off > 0; off >>= 1) {
+    if (c < off) {
+      uint64_t other[L];
+#pragma unroll
+      for (int i = 0; i < L; i++) other[i] = red[lc][c + off][i];
+#pragma unroll
+      for (int i = 0; i < L; i++) acc[i] = red[lc][c][i];
+      add_mod<L>(acc, other);
+#pragma unroll
+      for (int i = 0; i < L; i++) red[lc][c][i] = acc[i];
+    }
+    __syncthreads();
+  }
+  if (c == 0 && col < row_len) {
+#pragma unroll
+    for (int i = 0; i < L; i++) out[col * L + i] = red[lc][0][i];
+  }
 }
 
 template <int L>
 void launch_final(const uint64_t* part, const int64_t* alphas, uint64_t* out, uint64_t row_len,
-                  uint32_t D, uint32_t batch, hipStream_t s) {
-  hipLaunchKernelGGL((comb_final_kernel<L>), dim3((uint32_t)((row_len + 255) / 256)), dim3(256),
-                     0, s, part, alphas, out, row_len, D, batch);
+                  uint32_t D, uint32_t nseg, uint32_t batch, hipStream_t s) {
+  const uint32_t blocks = (uint32_t)((row_len + kColsPerBlock - 1) / kColsPerBlock);
+  hipLaunchKernelGGL((comb_final_kernel<L>), dim3(blocks), dim3(64 * kColsPerBlock), 0, s, part,
+                     alphas, out, row_len, D, nseg, batch);
+}
+
+uint32_t num_segments(uint64_t num_rows) {
+  return (uint32_t)((num_rows + kSegRows - 1) / kSegRows);
 }
 
 }  // namespace
 
-uint64_t combined_row_scratch_bytes(uint32_t D, uint64_t row_len, uint32_t batch) {
-  return (uint64_t)batch * D * row_len * 3 * sizeof(uint64_t);
+uint64_t combined_row_scratch_bytes(uint32_t D, uint64_t row_len, uint32_t batch,
+                                    uint64_t num_rows) {
+  return (uint64_t)batch * num_segments(num_rows) * D * row_len * 3 * sizeof(uint64_t);
 }
 
 int launch_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
                         uint32_t batch, const uint64_t* evals, const int64_t* alphas,
                         const int64_t* coeffs, uint64_t* out, void* scratch, hipStream_t s) {
-  const uint64_t total = (uint64_t)batch * D * row_len;
-  if (total == 0) return ZP_OK;
+  if ((uint64_t)batch * D * row_len == 0) return ZP_OK;
   uint64_t* part = (uint64_t*)scratch;
+  const uint32_t ngroups = (D + 7) / 8, nseg = num_segments(num_rows);
+  const uint64_t total = (uint64_t)batch * ngroups * nseg * row_len;
   hipLaunchKernelGGL(comb_partial_kernel, dim3((uint32_t)((total + 255) / 256)), dim3(256), 0, s,
-                     evals, coeffs, part, num_rows, row_len, D, batch);
+                     evals, coeffs, part, num_rows, row_len, D, ngroups, nseg, batch);
   ZP_HIP(hipGetLastError());
   switch (L) {
-    case 3: launch_final<3>(part, alphas, out, row_len, D, batch, s); break;
-    case 4: launch_final<4>(part, alphas, out, row_len, D, batch, s); break;
-    case 5: launch_final<5>(part, alphas, out, row_len, D, batch, s); break;
-    case 6: launch_final<6>(part, alphas, out, row_len, D, batch, s); break;
-    case 7: launch_final<7>(part, alphas, out, row_len, D, batch, s); break;
-    case 8: launch_final<8>(part, alphas, out, row_len, D, batch, s); break;
+    case 3: launch_final<3>(part, alphas, out, row_len, D, nseg, batch, s); break;
+    case 4: launch_final<4>(part, alphas, out, row_len, D, nseg, batch, s); break;
+    case 5: launch_final<5>(part, alphas, out, row_len, D, nseg, batch, s); break;
+    case 6: launch_final<6>(part, alphas, out, row_len, D, nseg, batch, s); break;
+    case 7: launch_final<7>(part, alphas, out, row_len, D, nseg, batch, s); break;
+    case 8: launch_final<8>(part, alphas, out, row_len, D, nseg, batch, s); break;
     default: return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
   }
   ZP_HIP(hipGetLastError());

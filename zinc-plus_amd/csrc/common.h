@@ -62,6 +62,10 @@ void dev_free(void* p, hipStream_t s);
 void dev_release_cache();
 uint8_t* pinned_scratch(size_t bytes);
 
+// Tunables (set through zp_set_option / the autotuner).
+extern int g_raa_wide;
+extern int g_raa_planes;
+
 // Host-side tables (params.cpp).
 int iprs_params(uint64_t row_len, uint64_t depth, uint64_t rep, uint32_t* bl, uint32_t* bd);
 uint64_t iprs_optimal_depth(uint64_t row_len);
@@ -94,6 +98,7 @@ int launch_merkle_layers(uint8_t* layers, uint64_t n, hipStream_t s);
 int launch_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
                         uint32_t batch, const uint64_t* evals, const int64_t* alphas,
                         const int64_t* coeffs, uint64_t* out, void* scratch, hipStream_t s);
-uint64_t combined_row_scratch_bytes(uint32_t D, uint64_t row_len, uint32_t batch);
+uint64_t combined_row_scratch_bytes(uint32_t D, uint64_t row_len, uint32_t batch,
+                                    uint64_t num_rows);
 
 }  // namespace zpi

@@ -6,75 +6,395 @@
 // after the repeat and after the first accumulate, :135-165).
 //
 // Per coefficient plane p of a row with codeword length N:
-//   y1[i] = s1(perm1[i]) * bit_p(x[perm1[i] mod row_len])
-//   a[i]  = s2(i) * sum_{l <= i} y1[l]
-//   out[i] = sum_{l <= i} a[perm2[i]]
-// with s1, s2 = -1 at odd indices for the sign-flipping code, else +1.
-// The workgroup keeps a[] in LDS (the only randomly gathered array) and
-// computes both scans without materialising y1 or the second gather.
-// Values stay below 2^26 in magnitude (raa.rs:52-69 width check), and the
-// arithmetic wraps like the reference's i32 accumulate_unchecked.
+//   y1[i]  = s(perm1[i]) * bit_p(x[perm1[i] mod row_len])
+//   a[i]   = s(i) * sum_{l <= i} y1[l]
+//   out[i] = sum_{l <= i} a[perm2[l]]
+// where s(i) = -1 for odd i under the sign-flipping code and +1 otherwise.
+//
+// A workgroup owns one row and a group of up to 8 planes.  It keeps a[] in
+// LDS as 16-bit values (|a| <= N for the plain code, <= N/2 + 1 with sign
+// flips, so u16 / i16 hold them for N <= 2^15) and never materialises y1 or
+// the gathered sequence: each scan is a per-thread segment sum, a block-wide
+// exclusive scan of the segment sums, and a second pass that emits values.
+// Output values stay below 2^26 in magnitude (raa.rs:52-69) and wrap like
+// the reference's i32 accumulate_unchecked.  Each position of the output
+// row is 4*D contiguous bytes; a workgroup writes its planes' 4*PG bytes,
+// and the plane groups of one row run on one XCD so the pieces of each line
+// meet in the same L2.
 #include "common.h"
 
 namespace zpi {
 namespace {
 
-// Inclusive-to-exclusive block scan of one value per thread (blockDim <= 1024).
-__device__ int32_t block_exclusive_scan(int32_t v, int32_t* part) {
-  const uint32_t t = threadIdx.x, T = blockDim.x;
-  part[t] = v;
+constexpr int kThreads = 512;
+
+}  // namespace
+
+// 8 planes per workgroup at N = 8192 (one workgroup per CU) instead of 4
+// (two per CU) in the plane-group kernel.
+int g_raa_wide = 1;
+// Forces the plane-group kernel where the full-row kernel would run.
+int g_raa_planes = 0;
+
+namespace {
+
+// Lane-wise 16-bit add / negate on packed pairs (no carry between lanes).
+__device__ __forceinline__ uint32_t add16x2(uint32_t a, uint32_t b) {
+  return ((a & 0xffff0000u) + (b & 0xffff0000u)) | ((a + b) & 0xffffu);
+}
+__device__ __forceinline__ uint32_t neg16x2(uint32_t a) { return add16x2(~a, 0x00010001u); }
+
+// Exclusive scan over the block of K per-thread words; kPacked words hold
+// two independent 16-bit lanes.  part has kThreads * K words.
+template <int K, bool kPacked>
+__device__ void block_exscan(uint32_t v[K], uint32_t* part) {
+  const uint32_t t = threadIdx.x;
+#pragma unroll
+  for (int k = 0; k < K; k++) part[k * kThreads + t] = v[k];
   __syncthreads();
-  for (uint32_t off = 1; off < T; off <<= 1) {
-    int32_t add = t >= off ? part[t - off] : 0;
+  for (uint32_t off = 1; off < kThreads; off <<= 1) {
+    uint32_t add[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) add[k] = t >= off ? part[k * kThreads + t - off] : 0u;
     __syncthreads();
-    part[t] = (int32_t)((uint32_t)part[t] + (uint32_t)add);
+#pragma unroll
+    for (int k = 0; k < K; k++)
+      part[k * kThreads + t] = kPacked ? add16x2(part[k * kThreads + t], add[k])
+                                       : part[k * kThreads + t] + add[k];
     __syncthreads();
   }
-  const int32_t incl = part[t];
+#pragma unroll
+  for (int k = 0; k < K; k++)
+    v[k] = kPacked ? add16x2(part[k * kThreads + t], neg16x2(v[k])) : part[k * kThreads + t] - v[k];
   __syncthreads();
-  return (int32_t)((uint32_t)incl - (uint32_t)v);
 }
 
-__global__ void __launch_bounds__(256)
+// 2*NW plane bits of `b` as NW pairs of 16-bit lanes.
+template <int NW>
+__device__ __forceinline__ void spread_pairs(uint32_t b, uint32_t out[NW]) {
+#pragma unroll
+  for (int m = 0; m < NW; m++)
+    out[m] = ((b >> (2 * m)) & 1u) | (((b >> (2 * m + 1)) & 1u) << 16);
+}
+
+template <int NW> struct Vec;
+template <> struct Vec<4> { typedef uint4 T; };
+template <> struct Vec<2> { typedef uint2 T; };
+template <> struct Vec<1> { typedef uint32_t T; };
+template <int NW> __device__ __forceinline__ void unpack(const typename Vec<NW>::T& v, uint32_t w[NW]);
+template <> __device__ __forceinline__ void unpack<4>(const uint4& v, uint32_t w[4]) {
+  w[0] = v.x; w[1] = v.y; w[2] = v.z; w[3] = v.w;
+}
+template <> __device__ __forceinline__ void unpack<2>(const uint2& v, uint32_t w[2]) {
+  w[0] = v.x; w[1] = v.y;
+}
+template <> __device__ __forceinline__ void unpack<1>(const uint32_t& v, uint32_t w[1]) { w[0] = v; }
+template <int NW> __device__ __forceinline__ typename Vec<NW>::T pack(const uint32_t w[NW]);
+template <> __device__ __forceinline__ uint4 pack<4>(const uint32_t w[4]) {
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+template <> __device__ __forceinline__ uint2 pack<2>(const uint32_t w[2]) {
+  return make_uint2(w[0], w[1]);
+}
+template <> __device__ __forceinline__ uint32_t pack<1>(const uint32_t w[1]) { return w[0]; }
+
+template <bool kSigned>
+__device__ __forceinline__ uint32_t lo16(uint32_t v) {
+  return kSigned ? (uint32_t)(int32_t)(int16_t)(v & 0xffffu) : (v & 0xffffu);
+}
+template <bool kSigned>
+__device__ __forceinline__ uint32_t hi16(uint32_t v) {
+  return kSigned ? (uint32_t)((int32_t)v >> 16) : (v >> 16);
+}
+
+// S = positions per thread (N / kThreads, at least 1); all per-position
+// inputs of a thread are loaded into registers once and reused by both the
+// summing and the emitting pass of each scan.
+template <int NW, int S, bool kFlip>
+__global__ void __launch_bounds__(kThreads)
 raa_binary_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
                   const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
-                  uint32_t row_len, uint32_t N, uint32_t D, int sign_flip, uint64_t row0) {
-  extern __shared__ int32_t a[];  // N values
-  __shared__ int32_t part[256];
-  const uint32_t t = threadIdx.x, T = blockDim.x;
-  const uint32_t p = blockIdx.x;
-  const uint64_t row = row0 + blockIdx.y;
+                  uint32_t row_len, uint32_t N, uint32_t D, uint32_t npg, uint64_t rows,
+                  uint64_t row0) {
+  constexpr uint32_t PG = 2 * NW;  // planes per workgroup
+  typedef typename Vec<NW>::T V;
+  extern __shared__ uint32_t lds[];
+  V* A = (V*)lds;                           // N x (PG planes x 16 bit)
+  uint32_t* part = lds + NW * (size_t)N;    // kThreads x PG scan lanes
+  const uint32_t t = threadIdx.x;
+
+  // Block -> (row, plane group); the groups of one row share an XCD.
+  const uint64_t b = blockIdx.x;
+  const uint64_t q = b >> 3;
+  const uint64_t row = row0 + (b & 7) + 8 * (q / npg);
+  const uint32_t pg = (uint32_t)(q % npg);
+  if (row >= rows) return;
+  const uint32_t p0 = PG * pg;
+  const uint32_t pmask = (1u << PG) - 1u;
   const uint64_t* x = evals + row * row_len;
-  const uint32_t S = (N + T - 1) / T;
-  const uint32_t i0 = min(N, t * S), i1 = min(N, i0 + S);
+  const uint32_t i0 = t * S;
+  const bool active = i0 < N;  // N < kThreads leaves some threads idle
   const uint32_t rmask = row_len - 1;
 
-  // First accumulate.
-  uint32_t sum = 0;
-  for (uint32_t i = i0; i < i1; i++) {
-    const uint32_t src = perm1[i];
-    uint32_t v = (uint32_t)(x[src & rmask] >> p) & 1u;
-    if (sign_flip && (src & 1u)) v = 0u - v;
-    sum += v;
+  // Prefetch the plane bits of every position (4 per word), the parity of
+  // each perm1 entry (one mask) and the perm2 entries (two 16-bit per word).
+  constexpr int SB = (S + 3) / 4, SP = (S + 1) / 2;
+  uint32_t bitw[SB], p2w[SP];
+  uint64_t odd = 0;
+  {
+    uint32_t src[S];
+#pragma unroll
+    for (int k = 0; k < S; k++) src[k] = active ? perm1[i0 + k] : 0u;
+#pragma unroll
+    for (int k = 0; k < SB; k++) bitw[k] = 0;
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      const uint32_t bv = active ? (uint32_t)(x[src[k] & rmask] >> p0) & pmask : 0u;
+      bitw[k / 4] |= bv << (8 * (k % 4));
+      odd |= (uint64_t)(src[k] & 1u) << k;
+    }
+#pragma unroll
+    for (int k = 0; k < SP; k++) {
+      const uint32_t a = active ? perm2[i0 + 2 * k] : 0u;
+      const uint32_t c = (active && 2 * k + 1 < S) ? perm2[i0 + 2 * k + 1] : 0u;
+      p2w[k] = a | (c << 16);
+    }
   }
-  uint32_t run = (uint32_t)block_exclusive_scan((int32_t)sum, part);
-  for (uint32_t i = i0; i < i1; i++) {
-    const uint32_t src = perm1[i];
-    uint32_t v = (uint32_t)(x[src & rmask] >> p) & 1u;
-    if (sign_flip && (src & 1u)) v = 0u - v;
-    run += v;
-    a[i] = (int32_t)((sign_flip && (i & 1u)) ? 0u - run : run);
+#define BITS(k) ((bitw[(k) / 4] >> (8 * ((k) % 4))) & 0xffu)
+#define ODD(k) ((odd >> (k)) & 1u)
+#define P2(k) ((p2w[(k) / 2] >> (16 * ((k) % 2))) & 0xffffu)
+
+  // First accumulate over y1 (NW packed 16-bit pairs = PG planes).
+  uint32_t sum[PG];
+#pragma unroll
+  for (int k = 0; k < (int)PG; k++) sum[k] = 0;
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    uint32_t v[NW];
+    spread_pairs<NW>(BITS(k), v);
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      if (kFlip && ODD(k)) v[m] = neg16x2(v[m]);
+      sum[m] = add16x2(sum[m], v[m]);
+    }
+  }
+  block_exscan<NW, true>(sum, part);
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < S; k++) {
+      uint32_t v[NW];
+      spread_pairs<NW>(BITS(k), v);
+      uint32_t a[NW];
+#pragma unroll
+      for (int m = 0; m < NW; m++) {
+        if (kFlip && ODD(k)) v[m] = neg16x2(v[m]);
+        sum[m] = add16x2(sum[m], v[m]);
+        a[m] = (kFlip && ((i0 + k) & 1u)) ? neg16x2(sum[m]) : sum[m];
+      }
+      A[i0 + k] = pack<NW>(a);
+    }
   }
   __syncthreads();
 
-  // Second accumulate over the perm2 gather.
-  sum = 0;
-  for (uint32_t i = i0; i < i1; i++) sum += (uint32_t)a[perm2[i]];
-  run = (uint32_t)block_exclusive_scan((int32_t)sum, part);
-  int32_t* o = out + (row * N) * D + p;
+  // Second accumulate over the perm2 gather, in 32-bit lanes.
+  uint32_t g[S][NW];
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+    if (active) {
+      unpack<NW>(A[P2(k)], g[k]);
+    } else {
+#pragma unroll
+      for (int m = 0; m < NW; m++) g[k][m] = 0;
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < (int)PG; k++) sum[k] = 0;
+#pragma unroll
+  for (int k = 0; k < S; k++)
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      sum[2 * m] += lo16<kFlip>(g[k][m]);
+      sum[2 * m + 1] += hi16<kFlip>(g[k][m]);
+    }
+  block_exscan<PG, false>(sum, part);
+  if (!active) return;
+  const uint32_t np = min(PG, D - p0);
+  int32_t* o = out + (row * N + i0) * D + p0;
+  const bool vec = np == PG && PG >= 4 && (D & 3) == 0;
+#pragma unroll
+  for (int k = 0; k < S; k++) {
+#pragma unroll
+    for (int m = 0; m < NW; m++) {
+      sum[2 * m] += lo16<kFlip>(g[k][m]);
+      sum[2 * m + 1] += hi16<kFlip>(g[k][m]);
+    }
+    int32_t* dst = o + (uint64_t)k * D;
+    if (vec) {
+#pragma unroll
+      for (int c = 0; c < (int)PG; c += 4)
+        *(uint4*)(dst + c) = make_uint4(sum[c], sum[c + 1], sum[c + 2], sum[c + 3]);
+    } else {
+      for (uint32_t c = 0; c < np; c++) dst[c] = (int32_t)sum[c];
+    }
+  }
+#undef BITS
+#undef ODD
+#undef P2
+}
+
+template <int NW, int S, bool kFlip>
+int launch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D, int32_t* out,
+               hipStream_t s) {
+  const uint32_t N = (uint32_t)cd->N;
+  const size_t lds = (size_t)N * 4 * NW + (size_t)kThreads * 2 * NW * 4;
+  const void* fn = (const void*)raa_binary_kernel<NW, S, kFlip>;
+  if (lds > 64 * 1024)
+    ZP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  const uint32_t npg = (D + 2 * NW - 1) / (2 * NW);
+  // Blocks are grouped 8 rows at a time: block = 8*(rowgroup*npg + pg) + r8.
+  const uint64_t max_groups = std::max<uint64_t>(1, 0x7fffffffull / (8ull * npg));
+  for (uint64_t r0 = 0; r0 < rows; r0 += 8 * max_groups) {
+    const uint64_t nr = std::min<uint64_t>(8 * max_groups, rows - r0);
+    const uint64_t groups = (nr + 7) / 8;
+    const uint32_t blocks = (uint32_t)(groups * npg * 8);
+    hipLaunchKernelGGL((raa_binary_kernel<NW, S, kFlip>), dim3(blocks), dim3(kThreads), lds, s,
+                       evals, out, cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, D, npg,
+                       r0 + nr, r0);
+    ZP_HIP(hipGetLastError());
+  }
+  return ZP_OK;
+}
+
+// Full-row kernel for the plain code with D = 64 or 32 planes.
+//
+// The first prefix sum of plane p is a count of set bits of y1, so it is
+// stored compactly: the y1 bit-vector of every plane (N bits) plus a 16-bit
+// checkpoint per 64 positions (the count before that word):
+//   a[m][p] = cp[p][m / 64] + popcount(bits[p][m / 64] & ones(m % 64 + 1)).
+// A workgroup holds all planes of one row (64 planes, N = 8192: 64 KiB of
+// bits + 16 KiB of checkpoints).  A checkpoint per 128 positions would let
+// two workgroups share a CU but costs a third lookup per rank, which
+// measured slower.  Phase 1 puts
+// positions in lanes and turns each plane's bits into 64-bit words with one
+// ballot per plane; phase 2 puts planes in lanes, so each output position
+// (4*D bytes) is written by one store instruction.
+constexpr uint32_t kRowThreads = 512;
+constexpr uint32_t kRowWaves = kRowThreads / 64;
+
+template <int D>
+__global__ void __launch_bounds__(kRowThreads)
+raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
+               const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
+               uint32_t row_len, uint32_t N, uint64_t row0) {
+  constexpr uint32_t SUB = 64 / D;           // position streams per wave
+  constexpr uint32_t NS = kRowWaves * SUB;   // position streams per workgroup
+  extern __shared__ uint64_t lds64[];
+  const uint32_t W = N / 64;                 // 64-bit words per plane
+  const uint32_t WP = W + 1;                 // padded plane stride (words)
+  const uint32_t CP = W + 1;                 // padded checkpoint stride
+  uint64_t* bits = lds64;                                  // [D][WP]
+  uint16_t* cp = (uint16_t*)(lds64 + (size_t)D * WP);      // [D][CP]
+  uint32_t* part = (uint32_t*)(lds64 + (size_t)D * WP + ((size_t)D * CP + 3) / 4);
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t row = row0 + blockIdx.x;
+  const uint64_t* x = evals + row * row_len;
+  const uint32_t rmask = row_len - 1;
+
+  // Phase 1a: y1 bit-vectors, 64 positions per wave step.
+  for (uint32_t blk = wave; blk < W; blk += kRowWaves) {
+    const uint32_t src = perm1[blk * 64 + lane];
+    const uint64_t xv = x[src & rmask];
+#pragma unroll 8
+    for (int p = 0; p < D; p++) {
+      const uint64_t m = __ballot((xv >> p) & 1ull);
+      if (lane == 0) bits[p * WP + blk] = m;
+    }
+  }
+  __syncthreads();
+
+  // Phase 1b: checkpoints (exclusive counts per word), one thread per
+  // (plane, segment of words) plus a scan over the segments.
+  {
+    constexpr uint32_t SEGS = kRowThreads / D;
+    const uint32_t p = tid % D, seg = tid / D;
+    const uint32_t ws = (W + SEGS - 1) / SEGS;
+    const uint32_t w0 = min(W, seg * ws), w1 = min(W, w0 + ws);
+    uint32_t cnt = 0;
+    for (uint32_t w = w0; w < w1; w++) cnt += __popcll(bits[p * WP + w]);
+    part[tid] = cnt;
+    __syncthreads();
+    uint32_t before = 0;
+    for (uint32_t sg = 0; sg < seg; sg++) before += part[sg * D + p];
+    for (uint32_t w = w0; w < w1; w++) {
+      cp[p * CP + w] = (uint16_t)before;
+      before += __popcll(bits[p * WP + w]);
+    }
+  }
+  __syncthreads();
+
+  // Phase 2: second accumulate, planes in lanes.  Stream s covers positions
+  // [s*L, (s+1)*L); pass 1 sums, the streams exchange carries, pass 2 emits.
+  const uint32_t sub = lane / D, p = lane % D;
+  const uint32_t s = wave * SUB + sub;
+  const uint32_t L = N / NS;
+  const uint32_t i0 = s * L, i1 = i0 + L;
+  const uint64_t* bp = bits + p * WP;
+  const uint16_t* cpp = cp + p * CP;
+  auto rank = [&](uint32_t m) -> uint32_t {
+    const uint64_t mask = (2ull << (m & 63)) - 1ull;  // ones(m % 64 + 1)
+    return cpp[m >> 6] + __popcll(bp[m >> 6] & mask);
+  };
+  uint32_t sum = 0;
+  for (uint32_t i = i0; i < i1; i++) sum += rank(perm2[i]);
+  __syncthreads();
+  part[s * D + p] = sum;
+  __syncthreads();
+  uint32_t run = 0;
+  for (uint32_t sg = 0; sg < s; sg++) run += part[sg * D + p];
+  int32_t* o = out + (row * N + i0) * D + p;
   for (uint32_t i = i0; i < i1; i++) {
-    run += (uint32_t)a[perm2[i]];
-    o[(uint64_t)i * D] = (int32_t)run;
+    run += rank(perm2[i]);
+    o[(uint64_t)(i - i0) * D] = (int32_t)run;
+  }
+}
+
+size_t raa_row_lds(uint32_t D, uint32_t N) {
+  const size_t WP = N / 64 + 1, CP = N / 64 + 1;
+  return (size_t)D * WP * 8 + (((size_t)D * CP + 3) / 4) * 8 + (size_t)kRowThreads * 4;
+}
+
+template <int D>
+int launch_raa_row(const zp_code* cd, const uint64_t* evals, uint64_t rows, int32_t* out,
+                   hipStream_t s) {
+  const uint32_t N = (uint32_t)cd->N;
+  const size_t lds = raa_row_lds(D, N);
+  if (lds > 64 * 1024)
+    ZP_HIP(hipFuncSetAttribute((const void*)raa_row_kernel<D>,
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  for (uint64_t r0 = 0; r0 < rows; r0 += 0x7fffffffull) {
+    const uint32_t nr = (uint32_t)std::min<uint64_t>(0x7fffffffull, rows - r0);
+    hipLaunchKernelGGL((raa_row_kernel<D>), dim3(nr), dim3(kRowThreads), lds, s, evals, out,
+                       cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, r0);
+    ZP_HIP(hipGetLastError());
+  }
+  return ZP_OK;
+}
+
+template <bool kFlip>
+int dispatch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
+                 int32_t* out, hipStream_t s) {
+  // Planes per workgroup: the 16-bit first-scan array (N x planes x 2 B)
+  // and the scan scratch must fit in 160 KiB of LDS.
+  switch (cd->N) {
+    case 32768: return launch_raa<1, 64, kFlip>(cd, evals, rows, D, out, s);
+    case 16384: return launch_raa<2, 32, kFlip>(cd, evals, rows, D, out, s);
+    case 8192:
+      return g_raa_wide ? launch_raa<4, 16, kFlip>(cd, evals, rows, D, out, s)
+                        : launch_raa<2, 16, kFlip>(cd, evals, rows, D, out, s);
+    case 4096: return launch_raa<4, 8, kFlip>(cd, evals, rows, D, out, s);
+    case 2048: return launch_raa<4, 4, kFlip>(cd, evals, rows, D, out, s);
+    case 1024: return launch_raa<4, 2, kFlip>(cd, evals, rows, D, out, s);
+    default: return launch_raa<4, 1, kFlip>(cd, evals, rows, D, out, s);  // N <= 512
   }
 }
 
@@ -83,21 +403,18 @@ raa_binary_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
 int launch_raa_binary(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
                       int32_t* out, hipStream_t s) {
   if (rows == 0) return ZP_OK;
-  const uint32_t N = (uint32_t)cd->N;
-  const size_t lds = (size_t)N * sizeof(int32_t);
-  if (lds > 128 * 1024)
+  if (cd->N > (1u << 15))
     return fail(ZP_ERR_INVALID_PARAM, "RAA codeword length above 32768 is not supported");
-  if (lds > 64 * 1024)
-    ZP_HIP(hipFuncSetAttribute((const void*)raa_binary_kernel,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-  for (uint64_t r0 = 0; r0 < rows; r0 += 65535) {
-    const uint32_t nr = (uint32_t)std::min<uint64_t>(65535, rows - r0);
-    hipLaunchKernelGGL(raa_binary_kernel, dim3(D, nr), dim3(256), lds, s, evals, out,
-                       cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, D,
-                       cd->kind == ZP_CODE_RAA_SIGNFLIP ? 1 : 0, r0);
-    ZP_HIP(hipGetLastError());
+  const uint32_t N = (uint32_t)cd->N;
+  // The full-row kernel: plain code, 32 or 64 planes, every stream a whole
+  // number of positions, and the bit-vectors within LDS.
+  if (cd->kind == ZP_CODE_RAA && !g_raa_planes && (D == 64 || D == 32) && N >= 1024 &&
+      raa_row_lds(D, N) <= 150 * 1024) {
+    return D == 64 ? launch_raa_row<64>(cd, evals, rows, out, s)
+                   : launch_raa_row<32>(cd, evals, rows, out, s);
   }
-  return ZP_OK;
+  return cd->kind == ZP_CODE_RAA_SIGNFLIP ? dispatch_raa<true>(cd, evals, rows, D, out, s)
+                                          : dispatch_raa<false>(cd, evals, rows, D, out, s);
 }
 
 }  // namespace zpi
