@@ -61,16 +61,17 @@ int zp_init(int device);
 const char* zp_status_string(int status);
 const char* zp_last_error(void);
 
-/* Library options (no reference counterpart):
+/* Library-wide overrides of the kernel variants (no reference counterpart).
+ * -1 (the initial value) leaves the choice to each code's tuned variant
+ * (zp_autotune) or, for an untuned code, the default (variant 0 / 1 below).
  *   "iprs_encoder": 0 = matrix-core encoder where eligible (default),
  *                   1 = VALU encoder for every shape.
- *   "raa_plane_groups": 1 = RAA rows by groups of planes for every shape,
- *                   0 = the full-row kernel where eligible (default).
- *   "raa_wide_groups": plane-group kernel at codeword length 8192 with 8
- *                   planes per workgroup (1, default) or 4 (0).
+ *   "raa_kernel":   0 = full-row kernel where eligible (default),
+ *                   1 = plane groups (8 planes per workgroup at codeword
+ *                       length 8192), 2 = plane groups (4 planes at 8192).
  *   "commit_overlap": 1 = hash finished row groups on a second stream while
  *                   later rows encode (default), 0 = encode, then hash.
- * Returns ZP_ERR_INVALID_PARAM for an unknown name. */
+ * Returns ZP_ERR_INVALID_PARAM for an unknown name or value. */
 int zp_set_option(const char* name, int64_t value);
 
 /* ---- LinearCode (zip-plus/src/code.rs:10-72) -------------------------- */
@@ -131,6 +132,55 @@ int zp_merkle_prove(const zp_hint* hint, uint64_t leaf, uint8_t* h_siblings, uin
  * zip-plus/src/pcs/phase_prove.rs:247-304): batch*num_rows elements. */
 int zp_hint_open_column(const zp_hint* hint, uint64_t col, void* h_out, zp_stream stream);
 void zp_hint_free(zp_hint* hint);
+
+/* ---- Batched commitments ------------------------------------------------
+ * A plan of independent commitments, each one ZipPlus::commit
+ * (phase_commit.rs:61-140) with its own code, shape and inputs, as the
+ * reference's callers issue them one by one (protocol/src/prover.rs:710-726,
+ * zip-plus/benches/zip_common.rs:191-230).  The plan owns every hint (one
+ * device pool) and a fixed scratch arena sized by its first commit; later
+ * commits run from the arena, and with ZP_BATCH_GRAPH replay one HIP graph
+ * of the whole batch (recaptured only when the input addresses change).
+ * Results are those of zp_commit on each item. */
+
+typedef struct zp_batch zp_batch;
+
+typedef struct {
+  const zp_code* code;
+  int32_t eval_type;        /* zp_eval_type */
+  uint32_t degree_plus_one; /* D of BinaryPoly<D> (ignored for i32) */
+  uint64_t num_rows;
+  uint32_t batch; /* polys under this one root (1 = commit_single) */
+} zp_commit_desc;
+
+enum { ZP_BATCH_GRAPH = 1, ZP_BATCH_AUTOTUNE = 2 };
+
+/* flags: ZP_BATCH_GRAPH, ZP_BATCH_AUTOTUNE (run zp_autotune once per
+ * distinct code at creation).  n_streams: branch streams (0 = 4). */
+int zp_batch_create(const zp_commit_desc* items, uint32_t n_items, uint32_t flags,
+                    uint32_t n_streams, zp_batch** out);
+/* d_evals[i]: item i's evals (batch x num_rows x row_len, poly-major).
+ * h_roots: n_items x 32 bytes.  Synchronous.  The hints of the previous
+ * commit are overwritten; calls on one plan must not overlap. */
+int zp_batch_commit(zp_batch* plan, const void* const* d_evals, uint8_t* h_roots,
+                    zp_stream stream);
+/* Item i's hint (valid until the next commit or destroy of the plan;
+ * zp_hint_free on it does nothing). */
+zp_hint* zp_batch_hint(zp_batch* plan, uint32_t item);
+int zp_batch_info(const zp_batch* plan, uint64_t* hint_bytes, uint64_t* scratch_bytes,
+                  uint32_t* graph_ready);
+void zp_batch_destroy(zp_batch* plan);
+
+/* ---- Autotuner (no reference counterpart) --------------------------------
+ * Times each kernel variant of `code` on synthetic inputs of the given
+ * shape (encoder for IPRS binary polys; RAA kernel; commit overlap) and
+ * stores the fastest in the code; later encodes and commits use it.  Must
+ * not run concurrently with other uses of the code. */
+int zp_autotune(zp_code* code, int eval_type, uint32_t degree_plus_one, uint64_t num_rows,
+                uint32_t batch, zp_stream stream);
+/* The tuned variants (-1 = not tuned), numbered as in zp_set_option. */
+int zp_code_tuning(const zp_code* code, int32_t* iprs_encoder, int32_t* raa_kernel,
+                   int32_t* commit_overlap);
 
 /* ---- Prover combined row (zip-plus/src/pcs/phase_prove.rs:153-244) ----- */
 

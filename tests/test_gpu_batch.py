@@ -1,0 +1,196 @@
+"""Batched commitment plans and the autotuner, against the CPU oracle.
+
+Every root of a plan must equal the oracle's ZipPlus::commit of that item,
+on the first (eager, measuring) commit, on the commit that captures the HIP
+graph, and on later replays -- with ordinary eager commits run between the
+replays -- and again after the inputs move (which recaptures the graph).
+"""
+import numpy as np
+import pytest
+import torch
+
+from zipgpu import synth
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+
+
+def to_dev(a, dev):
+    return torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+
+
+# (kind, n, D, eval_type, batch): IPRS matrix-core and VALU shapes, i32
+# evals, RAA full-row and plane-group shapes, the sign-flipping code, and an
+# item with several polys under one root.
+ITEMS = [
+    ("iprs", 1 << 12, 32, "binpoly", 1),
+    ("iprs", 1 << 15, 64, "binpoly", 1),
+    ("raa", 1 << 12, 32, "binpoly", 1),
+    ("raa", 1 << 14, 64, "binpoly", 1),
+    ("iprs", 1 << 11, 0, "i32", 1),
+    ("raa_signflip", 1 << 12, 64, "binpoly", 1),
+    ("iprs", 1 << 10, 7, "binpoly", 3),
+    ("raa", 1 << 16, 32, "binpoly", 1),
+]
+
+
+def make_code(zg, kind, row_len):
+    if kind == "iprs":
+        return zg.IprsCode.new_with_optimal_depth(row_len)
+    return zg.RaaCode(row_len) if kind == "raa" else zg.RaaSignFlippingCode(row_len)
+
+
+def build(zg, items):
+    plan_items, meta = [], []
+    for kind, n, D, et, batch in items:
+        row_len = synth.row_split("iprs" if kind == "iprs" else "raa", n)
+        if kind == "iprs":
+            row_len = min(row_len, 1 << 12)  # several rows per tree
+        types = zg.IntTypes() if et == "i32" else zg.BinPolyTypes(D)
+        code = make_code(zg, kind, row_len)
+        pp = zg.ZipPlus(types).setup(n, code)
+        plan_items.append((pp, types, batch))
+        meta.append((kind, n, D, et, batch, row_len, pp.num_rows))
+    return plan_items, meta
+
+
+def inputs(meta, seed):
+    out = []
+    for i, (kind, n, D, et, batch, _, _) in enumerate(meta):
+        s = seed * 100 + i
+        out.append(synth.i32_evals(s, batch * n) if et == "i32"
+                   else synth.binary_polys(s, batch * n, D))
+    return out
+
+
+def oracle_roots(oracle, meta, xs):
+    roots = []
+    for (kind, n, D, et, batch, row_len, rows), x in zip(meta, xs):
+        ev = x if et == "i32" else x.view(np.uint64)
+        roots.append(oracle.commit(kind, ev, rows, row_len, batch=batch, D=max(D, 1),
+                                   eval_type=et)["root"])
+    return roots
+
+
+def eager_noise(zg, dev):
+    # Ordinary work between graph replays: a plain commit and a torch op.
+    x = synth.binary_polys(5, 1 << 12, 32)
+    zp = zg.ZipPlus(zg.BinPolyTypes(32))
+    pp = zp.setup(1 << 12, zg.RaaCode(64))
+    hint, _ = zp.commit(pp, [to_dev(x, dev)])
+    hint.free()
+    torch.ones(1 << 20, device=dev).cumsum(0)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_batch_commit_matches_oracle(zg, dev, oracle, graph):
+    say("batch plan", "graph" if graph else "eager")
+    plan_items, meta = build(zg, ITEMS)
+    plan = zg.BatchCommit(plan_items, graph=graph)
+    xs = inputs(meta, 1)
+    want = oracle_roots(oracle, meta, xs)
+    xd = [to_dev(x, dev) for x in xs]
+    for rnd in range(4):
+        say("commit", rnd)
+        got = plan.commit(xd)
+        assert [c.root for c in got] == want, f"commit {rnd}"
+        assert [c.batch_size for c in got] == [m[4] for m in meta]
+        eager_noise(zg, dev)
+    info = plan.info()
+    assert info["graph_ready"] == graph
+    assert info["hint_bytes"] > 0
+    # New inputs at new addresses: the graph is captured again.
+    xs2 = inputs(meta, 2)
+    want2 = oracle_roots(oracle, meta, xs2)
+    xd2 = [to_dev(x, dev) for x in xs2]
+    for rnd in range(2):
+        got = plan.commit(xd2)
+        assert [c.root for c in got] == want2, f"moved inputs, commit {rnd}"
+    # Back to the first inputs, contents changed in place.
+    for i, x in enumerate(xs2):
+        xd[i].copy_(to_dev(x, dev))
+    got = plan.commit(xd)
+    assert [c.root for c in got] == want2
+
+
+def test_batch_hints_match_oracle(zg, dev, oracle):
+    items = [("iprs", 1 << 13, 32, "binpoly", 1), ("raa", 1 << 12, 64, "binpoly", 2)]
+    plan_items, meta = build(zg, items)
+    plan = zg.BatchCommit(plan_items, graph=True)
+    xs = inputs(meta, 3)
+    xd = [to_dev(x, dev) for x in xs]
+    for _ in range(3):
+        plan.commit(xd)
+    for i, ((kind, n, D, et, batch, row_len, rows), x) in enumerate(zip(meta, xs)):
+        say("hint", i)
+        ref = oracle.commit(kind, x.view(np.uint64), rows, row_len, batch=batch, D=D,
+                            want_cw=True, want_layers=True)
+        hint = plan.hint(i)
+        per = ref["cw"].reshape(batch, -1)
+        for b in range(batch):
+            assert hint.codewords(b).tobytes() == per[b].tobytes()
+        N = hint.num_leaves
+        assert np.array_equal(hint.layer(0), ref["layers"][:N])
+        col = N // 3
+        proof = hint.prove(col)
+        assert len(proof.siblings) == N.bit_length() - 1
+        assert oracle.merkle_verify(ref["root"], hint.open_column(col), col, N,
+                                    proof.siblings)
+        hint.free()  # a plan's hint is not freed by its holder
+    got = plan.commit(xd)
+    assert got[0].root == oracle_roots(oracle, meta, xs)[0]
+
+
+def test_batch_plan_errors(zg, dev):
+    with pytest.raises(zg.ShapeError):
+        zg.BatchCommit([])
+    plan_items, meta = build(zg, ITEMS[:2])
+    plan = zg.BatchCommit(plan_items)
+    with pytest.raises(zg.ShapeError):
+        plan.commit([torch.zeros(8, dtype=torch.int64, device=dev)] * 2)
+    with pytest.raises(zg.ShapeError):
+        plan.commit([])
+
+
+@pytest.mark.parametrize("kind,row_len,D,rows", [
+    ("iprs", 1 << 14, 32, 8), ("iprs", 1 << 12, 64, 4), ("raa", 2048, 64, 16),
+    ("raa", 256, 32, 8)])
+def test_autotune_keeps_results(zg, dev, oracle, kind, row_len, D, rows):
+    say("autotune", kind, row_len, D, rows)
+    code = make_code(zg, kind, row_len)
+    types = zg.BinPolyTypes(D)
+    assert code.tuning() == {"iprs_encoder": -1, "raa_kernel": -1, "commit_overlap": -1}
+    t = code.autotune(types, rows)
+    if kind == "iprs":
+        assert t["iprs_encoder"] in (0, 1) and t["raa_kernel"] == -1
+    else:
+        assert t["raa_kernel"] in (0, 1, 2) and t["iprs_encoder"] == -1
+    assert t["commit_overlap"] in (0, 1)
+    n = rows * row_len
+    x = synth.binary_polys(900 + row_len, n, D)
+    xd = to_dev(x, dev)
+    got = code.encode_rows(xd, rows, types).cpu().numpy()
+    if kind == "iprs":
+        want = oracle.iprs_encode_binary(x.view(np.uint64), rows, row_len, -1, D)
+    else:
+        want = oracle.raa_encode_binary(x.view(np.uint64), rows, row_len, D)
+    assert np.array_equal(got, want)
+    zp = zg.ZipPlus(types)
+    _, comm = zp.commit(zp.setup(n, code), [xd])
+    assert comm.root == oracle.commit(kind, x.view(np.uint64), rows, row_len, D=D)["root"]
+
+
+def test_batch_autotune_flag(zg, dev, oracle):
+    plan_items, meta = build(zg, ITEMS[:4])
+    plan = zg.BatchCommit(plan_items, graph=True, autotune=True)
+    for pp, _, _ in plan_items:
+        assert pp.linear_code.tuning()["commit_overlap"] in (0, 1)
+    xs = inputs(meta, 4)
+    xd = [to_dev(x, dev) for x in xs]
+    want = oracle_roots(oracle, meta, xs)
+    for _ in range(3):
+        assert [c.root for c in plan.commit(xd)] == want

@@ -60,12 +60,13 @@ def test_iprs_device_encoders_agree(zg, dev, oracle, row_len, depth, D):
     code = zg.IprsCode(row_len, depth)
     types = zg.BinPolyTypes(D)
     xd = to_dev(x, dev)
-    fast = code.encode_rows(xd, rows, types).cpu().numpy()
-    zg.set_option("iprs_encoder", 1)
     try:
+        zg.set_option("iprs_encoder", 0)
+        fast = code.encode_rows(xd, rows, types).cpu().numpy()
+        zg.set_option("iprs_encoder", 1)
         valu = code.encode_rows(xd, rows, types).cpu().numpy()
     finally:
-        zg.set_option("iprs_encoder", 0)
+        zg.set_option("iprs_encoder", -1)
     want = oracle.iprs_encode_binary(x.view(np.uint64), rows, row_len, depth, D)
     assert np.array_equal(fast, want) and np.array_equal(valu, want)
 
@@ -125,7 +126,7 @@ def test_raa_binary_encode(zg, dev, oracle, row_len, D, rows, flip):
 
 
 @pytest.mark.parametrize("row_len,D,rows", [(2048, 64, 3), (1024, 32, 2), (256, 64, 2),
-                                             (4096, 32, 1), (512, 32, 4)])
+                                             (4096, 32, 1), (512, 32, 4), (2048, 32, 2)])
 def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows):
     # The full-row RAA kernel (compact first scan, all planes per workgroup)
     # and the plane-group kernel are separate device paths; both must equal
@@ -135,14 +136,16 @@ def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows):
     code = zg.RaaCode(row_len)
     types = zg.BinPolyTypes(D)
     xd = to_dev(x, dev)
-    full = code.encode_rows(xd, rows, types).cpu().numpy()
-    zg.set_option("raa_plane_groups", 1)
+    outs = []
     try:
-        groups = code.encode_rows(xd, rows, types).cpu().numpy()
+        for variant in (0, 1, 2):  # full row, plane groups of 8 / of 4 planes
+            zg.set_option("raa_kernel", variant)
+            outs.append(code.encode_rows(xd, rows, types).cpu().numpy())
     finally:
-        zg.set_option("raa_plane_groups", 0)
+        zg.set_option("raa_kernel", -1)
     want = oracle.raa_encode_binary(x.view(np.uint64), rows, row_len, D)
-    assert np.array_equal(full, want) and np.array_equal(groups, want)
+    for got in outs:
+        assert np.array_equal(got, want)
 
 
 def test_raa_width_assert(zg, dev):
@@ -243,12 +246,13 @@ def test_commit_overlap_matches_sequential(zg, dev, oracle, kind, n, row_len, D)
     zp = zg.ZipPlus(zg.BinPolyTypes(D))
     pp = zp.setup(n, code)
     xd = to_dev(x, dev)
-    _, on = zp.commit(pp, [xd])
-    zg.set_option("commit_overlap", 0)
     try:
+        zg.set_option("commit_overlap", 1)
+        _, on = zp.commit(pp, [xd])
+        zg.set_option("commit_overlap", 0)
         _, off = zp.commit(pp, [xd])
     finally:
-        zg.set_option("commit_overlap", 1)
+        zg.set_option("commit_overlap", -1)
     ref = oracle.commit(kind, x.view(np.uint64), pp.num_rows, row_len, D=D)
     assert on.root == off.root == ref["root"]
 

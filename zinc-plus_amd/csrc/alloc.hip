@@ -51,8 +51,28 @@ void release_device_locked(int dev) {
 
 }  // namespace
 
+// A bump arena (set by the caller for the current thread) replaces the
+// cache while a graph is being captured: no device allocation may happen
+// then, and the graph must keep using the same scratch addresses.  A
+// measuring arena counts the requests the cache serves.
+static thread_local Arena* t_arena = nullptr;
+
+void set_arena(Arena* a) { t_arena = a; }
+
 int dev_alloc(void** out, size_t bytes, hipStream_t s) {
   *out = nullptr;
+  if (t_arena) {
+    const size_t want = (bytes + 255) / 256 * 256;
+    if (t_arena->measuring) {  // pass through to the cache, count bytes
+      t_arena->used += want;
+    } else {
+      if (t_arena->used + want > t_arena->size)
+        return fail(ZP_ERR_OOM, "graph scratch arena exhausted");
+      *out = t_arena->base + t_arena->used;
+      t_arena->used += want;
+      return ZP_OK;
+    }
+  }
   int dev = 0;
   ZP_HIP(hipGetDevice(&dev));
   const size_t want = round_size(bytes);
@@ -80,6 +100,7 @@ int dev_alloc(void** out, size_t bytes, hipStream_t s) {
 
 void dev_free(void* p, hipStream_t s) {
   if (!p) return;
+  if (t_arena && !t_arena->measuring) return;  // arena memory is released with the arena
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_size.find(p);
   if (it == g_size.end()) return;

@@ -18,24 +18,26 @@ int fail(int status, const std::string& msg) {
 
 using namespace zpi;
 
-struct zp_hint {
-  int device = 0;
-  hipStream_t stream = nullptr;  // the commit's stream; buffers are freed on it
-  uint8_t* d_cw = nullptr;      // batch x num_rows x N elements, poly-major
-  uint8_t* d_layers = nullptr;  // (2N-1) x 32 bytes, leaves first, root last
-  uint64_t N = 0, num_rows = 0, elem = 0, per_poly = 0;
-  uint32_t batch = 0, height = 0;
-};
+namespace zpi {
+
+// Library-wide overrides of the kernel variants; -1 leaves the choice to
+// the code's tuned variant (zp_autotune) or the default.
+int g_force_iprs = -1, g_force_raa = -1, g_force_overlap = -1;
+
+
+uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D) {
+  if (c->kind == ZP_CODE_IPRS) {
+    if (eval_type == ZP_EVAL_I32) return 16;
+    if (eval_type == ZP_EVAL_BINPOLY) return 8ull * D;
+  } else if (eval_type == ZP_EVAL_BINPOLY) {
+    return 4ull * D;
+  }
+  return 0;
+}
+
+uint64_t eval_bytes_of(int eval_type) { return eval_type == ZP_EVAL_I32 ? 4 : 8; }
 
 namespace {
-
-bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
-
-// Selects the VALU IPRS encoder instead of the matrix-core one (set by
-// zp_set_option; used to cross-check the two device paths).
-int g_force_valu = 0;
-// Disables the two-stream overlap of encoding and hashing inside a commit.
-int g_no_overlap = 0;
 
 template <class T>
 int upload(const std::vector<T>& host, T** dev) {
@@ -56,20 +58,14 @@ void free_code(zp_code* c) {
   delete c;
 }
 
-uint64_t elem_bytes(const zp_code* c, int eval_type, uint32_t D) {
-  if (c->kind == ZP_CODE_IPRS) {
-    if (eval_type == ZP_EVAL_I32) return 16;
-    if (eval_type == ZP_EVAL_BINPOLY) return 8ull * D;
-  } else if (eval_type == ZP_EVAL_BINPOLY) {
-    return 4ull * D;
-  }
-  return 0;
-}
+int pick(int forced, int tuned, int dflt) { return forced >= 0 ? forced : tuned >= 0 ? tuned : dflt; }
+
+}  // namespace
 
 int check_types(const zp_code* c, int eval_type, uint32_t D) {
   if (eval_type == ZP_EVAL_BINPOLY && (D == 0 || D > 64))
     return fail(ZP_ERR_INVALID_PARAM, "BinaryPoly degree bound must be within 1..64");
-  if (elem_bytes(c, eval_type, D) == 0)
+  if (cw_elem_bytes_of(c, eval_type, D) == 0)
     return fail(ZP_ERR_INVALID_PARAM, "unsupported eval type for this code");
   if (c->kind != ZP_CODE_IPRS) {
     // raa.rs:52-69: 1 + 2*log2(row_len) + 2*log2(REP) bits must fit i32.
@@ -82,39 +78,42 @@ int check_types(const zp_code* c, int eval_type, uint32_t D) {
   return ZP_OK;
 }
 
-int encode(const zp_code* c, int eval_type, uint32_t D, const void* evals, uint64_t num_rows,
-           void* out, hipStream_t s) {
-  ZP_TRY(check_types(c, eval_type, D));
+// Encodes with the variant in force for this code (types already checked).
+int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
+                        uint64_t num_rows, void* out, hipStream_t s) {
   if (c->kind == ZP_CODE_IPRS) {
     if (eval_type == ZP_EVAL_BINPOLY) {
-      if (iprs_mfma_eligible(c, D) && !g_force_valu)
+      if (iprs_mfma_eligible(c, D) && pick(g_force_iprs, c->tune_iprs, 0) == 0)
         return launch_iprs_binary_mfma(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
       return launch_iprs_binary(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
     }
     return launch_iprs_i32(c, (const int32_t*)evals, num_rows, out, s);
   }
-  return launch_raa_binary(c, (const uint64_t*)evals, num_rows, D, (int32_t*)out, s);
+  return launch_raa_binary(c, (const uint64_t*)evals, num_rows, D, (int32_t*)out, s,
+                           pick(g_force_raa, c->tune_raa, 0));
 }
 
-uint64_t eval_bytes(int eval_type) { return eval_type == ZP_EVAL_I32 ? 4 : 8; }
+namespace {
 
 // Encode all rows of all polys, then hash the leaves.  For a single poly
 // whose rows split into whole 1 KiB BLAKE3 chunks, the rows are encoded in
 // groups on `s` while the chunks of finished groups are hashed on the
 // auxiliary stream; the streams join before the chunk trees are merged.
 int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
-                           uint64_t num_rows, uint32_t batch, uint64_t in_stride, zp_hint* h,
-                           hipStream_t s) {
+                           uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
+                           bool allow_overlap) {
   const uint64_t N = h->N, elem = h->elem, rows_all = num_rows * batch;
+  const uint64_t in_stride = num_rows * code->row_len * eval_bytes_of(eval_type);
   const uint64_t rpc = (elem <= 1024 && 1024 % elem == 0) ? 1024 / elem : 0;  // rows per chunk
   const uint64_t nchunks = leaf_chunks(rows_all, (uint32_t)elem);
   const uint32_t kGroups = 4;
-  const bool pipelined = batch == 1 && rpc && num_rows % (rpc * kGroups) == 0 &&
-                         nchunks >= kGroups && !g_no_overlap;
+  const bool pipelined = allow_overlap && pick(g_force_overlap, code->tune_overlap, 1) == 1 &&
+                         batch == 1 && rpc && num_rows % (rpc * kGroups) == 0 &&
+                         nchunks >= kGroups;
   if (!pipelined) {
     for (uint32_t b = 0; b < batch; b++)
-      ZP_TRY(encode(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride, num_rows,
-                    h->d_cw + b * h->per_poly, s));
+      ZP_TRY(encode_rows_variant(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride,
+                                 num_rows, h->d_cw + b * h->per_poly, s));
     return launch_leaf_hash(h->d_cw, rows_all, N, (uint32_t)elem, h->d_layers, s);
   }
   hipStream_t s2;
@@ -128,9 +127,10 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
       rc = fail(ZP_ERR_DEVICE, "event creation failed");
   const uint64_t rows_q = num_rows / kGroups, chunks_q = nchunks / kGroups;
   for (uint32_t q = 0; rc == ZP_OK && q < kGroups; q++) {
-    rc = encode(code, eval_type, D,
-                (const uint8_t*)d_evals + q * rows_q * code->row_len * eval_bytes(eval_type),
-                rows_q, h->d_cw + q * rows_q * N * elem, s);
+    rc = encode_rows_variant(
+        code, eval_type, D,
+        (const uint8_t*)d_evals + q * rows_q * code->row_len * eval_bytes_of(eval_type), rows_q,
+        h->d_cw + q * rows_q * N * elem, s);
     if (rc == ZP_OK && (hipEventRecord(ev[q], s) != hipSuccess ||
                         hipStreamWaitEvent(s2, ev[q], 0) != hipSuccess))
       rc = fail(ZP_ERR_DEVICE, "event record/wait failed");
@@ -151,6 +151,43 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
 }
 
 }  // namespace
+
+int hint_alloc(const zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uint32_t batch,
+               hipStream_t s, zp_hint** out) {
+  *out = nullptr;
+  if (batch == 0) return fail(ZP_ERR_SHAPE, "Batch must contain at least one polynomial");
+  if (num_rows == 0) return fail(ZP_ERR_SHAPE, "num_rows must be positive");
+  ZP_TRY(check_types(code, eval_type, D));
+  const uint64_t N = code->N;
+  if (!pow2(N)) return fail(ZP_ERR_SHAPE, "codeword length must be a power of two");
+  zp_hint* h = new zp_hint();
+  h->device = code->device;
+  h->N = N;
+  h->num_rows = num_rows;
+  h->elem = cw_elem_bytes_of(code, eval_type, D);
+  h->per_poly = num_rows * N * h->elem;
+  h->batch = batch;
+  h->height = (uint32_t)__builtin_ctzll(N) + 1;
+  h->stream = s;
+  int rc = dev_alloc((void**)&h->d_cw, h->per_poly * batch, s);
+  if (rc == ZP_OK) rc = dev_alloc((void**)&h->d_layers, (2 * N - 1) * 32, s);
+  if (rc != ZP_OK) {
+    zp_hint_free(h);
+    return rc;
+  }
+  *out = h;
+  return ZP_OK;
+}
+
+int commit_enqueue(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
+                   uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
+                   bool allow_overlap) {
+  ZP_TRY(commit_encode_and_hash(code, eval_type, D, d_evals, num_rows, batch, h, s,
+                                allow_overlap));
+  return launch_merkle_layers(h->d_layers, h->N, s);
+}
+
+}  // namespace zpi
 
 extern "C" {
 
@@ -175,24 +212,20 @@ const char* zp_last_error(void) { return g_last_error.c_str(); }
 
 int zp_set_option(const char* name, int64_t value) {
   if (!name) return fail(ZP_ERR_INVALID_PARAM, "null option name");
-  if (std::strcmp(name, "iprs_encoder") == 0) {
-    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "iprs_encoder is 0 or 1");
-    g_force_valu = (int)value;
-    return ZP_OK;
-  }
-  if (std::strcmp(name, "raa_wide_groups") == 0) {
-    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "raa_wide_groups is 0 or 1");
-    g_raa_wide = (int)value;
-    return ZP_OK;
-  }
-  if (std::strcmp(name, "raa_plane_groups") == 0) {
-    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "raa_plane_groups is 0 or 1");
-    g_raa_planes = (int)value;
-    return ZP_OK;
-  }
-  if (std::strcmp(name, "commit_overlap") == 0) {
-    if (value != 0 && value != 1) return fail(ZP_ERR_INVALID_PARAM, "commit_overlap is 0 or 1");
-    g_no_overlap = value ? 0 : 1;
+  struct Opt {
+    const char* name;
+    int* var;
+    int64_t max;
+  };
+  const Opt opts[] = {{"iprs_encoder", &g_force_iprs, 1},
+                      {"raa_kernel", &g_force_raa, 2},
+                      {"commit_overlap", &g_force_overlap, 1}};
+  for (const auto& o : opts) {
+    if (std::strcmp(name, o.name) != 0) continue;
+    if (value < -1 || value > o.max)
+      return fail(ZP_ERR_INVALID_PARAM, std::string(name) + " is -1 (automatic) or 0.." +
+                                            std::to_string(o.max));
+    *o.var = (int)value;
     return ZP_OK;
   }
   return fail(ZP_ERR_INVALID_PARAM, std::string("unknown option ") + name);
@@ -272,14 +305,15 @@ uint32_t zp_code_depth(const zp_code* c) { return c ? c->depth : 0; }
 int zp_code_kind_of(const zp_code* c) { return c ? c->kind : -1; }
 
 uint64_t zp_cw_elem_bytes(const zp_code* c, int eval_type, uint32_t D) {
-  return c ? elem_bytes(c, eval_type, D) : 0;
+  return c ? cw_elem_bytes_of(c, eval_type, D) : 0;
 }
 
 int zp_encode_rows(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
                    uint64_t num_rows, void* d_out, zp_stream stream) {
   if (!code) return fail(ZP_ERR_INVALID_PARAM, "null code");
   if (num_rows && (!d_evals || !d_out)) return fail(ZP_ERR_INVALID_PARAM, "null buffer");
-  return encode(code, eval_type, D, d_evals, num_rows, d_out, (hipStream_t)stream);
+  ZP_TRY(check_types(code, eval_type, D));
+  return encode_rows_variant(code, eval_type, D, d_evals, num_rows, d_out, (hipStream_t)stream);
 }
 
 int zp_merkle_root(const void* d_rows, uint64_t num_rows, uint64_t width, uint32_t elem,
@@ -312,29 +346,12 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_eval
   hipStream_t s = (hipStream_t)stream;
   if (!code || !hint_out) return fail(ZP_ERR_INVALID_PARAM, "null argument");
   *hint_out = nullptr;
-  if (batch == 0) return fail(ZP_ERR_SHAPE, "Batch must contain at least one polynomial");
-  if (num_rows == 0) return fail(ZP_ERR_SHAPE, "num_rows must be positive");
-  ZP_TRY(check_types(code, eval_type, D));
-  const uint64_t N = code->N, elem = elem_bytes(code, eval_type, D);
-  if (!pow2(N)) return fail(ZP_ERR_SHAPE, "codeword length must be a power of two");
-  zp_hint* h = new zp_hint();
-  h->device = code->device;
-  h->N = N;
-  h->num_rows = num_rows;
-  h->elem = elem;
-  h->per_poly = num_rows * N * elem;
-  h->batch = batch;
-  h->height = (uint32_t)__builtin_ctzll(N) + 1;
-  h->stream = s;
-  int rc = dev_alloc((void**)&h->d_cw, h->per_poly * batch, s);
-  if (rc == ZP_OK) rc = dev_alloc((void**)&h->d_layers, (2 * N - 1) * 32, s);
+  zp_hint* h = nullptr;
+  ZP_TRY(hint_alloc(code, eval_type, D, num_rows, batch, s, &h));
   uint8_t* pin = pinned_scratch(32);
-  if (rc == ZP_OK && !pin) rc = fail(ZP_ERR_OOM, "pinned staging allocation failed");
-  const uint64_t in_stride = num_rows * code->row_len * eval_bytes(eval_type);
-  if (rc == ZP_OK) rc = commit_encode_and_hash(code, eval_type, D, d_evals, num_rows, batch,
-                                               in_stride, h, s);
-  if (rc == ZP_OK) rc = launch_merkle_layers(h->d_layers, N, s);
-  if (rc == ZP_OK && hipMemcpyAsync(pin, h->d_layers + (2 * N - 2) * 32, 32,
+  int rc = pin ? ZP_OK : fail(ZP_ERR_OOM, "pinned staging allocation failed");
+  if (rc == ZP_OK) rc = commit_enqueue(code, eval_type, D, d_evals, num_rows, batch, h, s, true);
+  if (rc == ZP_OK && hipMemcpyAsync(pin, h->d_layers + (2 * h->N - 2) * 32, 32,
                                     hipMemcpyDeviceToHost, s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "root copy failed");
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
@@ -412,7 +429,7 @@ int zp_hint_open_column(const zp_hint* h, uint64_t col, void* h_out, zp_stream s
 }
 
 void zp_hint_free(zp_hint* h) {
-  if (!h) return;
+  if (!h || !h->owned) return;
   dev_free(h->d_cw, h->stream);
   dev_free(h->d_layers, h->stream);
   delete h;

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <atomic>
 #include <string>
 
 #include "../../include/zipgpu.h"
@@ -51,20 +52,57 @@ struct zp_code {
   // RAA
   uint32_t* d_perm1 = nullptr;
   uint32_t* d_perm2 = nullptr;
+  // Kernel variants picked by the autotuner (-1 = not tuned: defaults).
+  mutable int tune_iprs = -1;     // 0 matrix-core encoder, 1 VALU encoder
+  mutable int tune_raa = -1;      // 0 full-row kernel, 1 plane groups (8), 2 plane groups (4)
+  mutable int tune_overlap = -1;  // commit: 1 overlap encode and hash on two streams, 0 not
+};
+
+// A commitment hint kept on the device (ZipPlusHint).
+struct zp_hint {
+  int device = 0;
+  hipStream_t stream = nullptr;  // buffers return to the cache of this stream
+  bool owned = true;             // false: buffers belong to a batch plan
+  uint8_t* d_cw = nullptr;       // batch x num_rows x N elements, poly-major
+  uint8_t* d_layers = nullptr;   // (2N-1) x 32 bytes, leaves first, root last
+  uint64_t N = 0, num_rows = 0, elem = 0, per_poly = 0;
+  uint32_t batch = 0, height = 0;
 };
 
 namespace zpi {
 
 // Device memory cache (alloc.hip): blocks are reused only on the stream
-// that freed them.
+// that freed them.  A thread-local arena, when set, serves requests from
+// its fixed block instead (fails when it is exhausted), or, when it is
+// measuring, lets the cache serve them and counts their rounded sizes.
+struct Arena {
+  uint8_t* base = nullptr;
+  size_t size = 0, used = 0;
+  bool measuring = false;
+};
+void set_arena(Arena* a);
 int dev_alloc(void** out, size_t bytes, hipStream_t s);
 void dev_free(void* p, hipStream_t s);
 void dev_release_cache();
 uint8_t* pinned_scratch(size_t bytes);
 
-// Tunables (set through zp_set_option / the autotuner).
-extern int g_raa_wide;
-extern int g_raa_planes;
+// Raises kernel K's dynamic-LDS limit to `lds` bytes the first time a launch
+// needs more than the 64 KiB default (not again afterwards, so that launches
+// recorded into a graph make no attribute calls).
+template <auto K>
+int grant_lds(size_t lds) {
+  static std::atomic<size_t> granted{64 * 1024};
+  if (lds <= granted.load()) return ZP_OK;
+  ZP_HIP(hipFuncSetAttribute((const void*)K, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  size_t g = granted.load();
+  while (g < lds && !granted.compare_exchange_weak(g, lds)) {
+  }
+  return ZP_OK;
+}
+
+// Library-wide overrides of the kernel variants (zp_set_option): -1 = use
+// the code's tuned or default choice.
+extern int g_force_iprs, g_force_raa, g_force_overlap;
 
 // Host-side tables (params.cpp).
 int iprs_params(uint64_t row_len, uint64_t depth, uint64_t rep, uint32_t* bl, uint32_t* bd);
@@ -83,7 +121,22 @@ int launch_iprs_binary_mfma(const zp_code* c, const uint64_t* evals, uint64_t ro
 int launch_iprs_i32(const zp_code* c, const int32_t* evals, uint64_t rows, void* out,
                     hipStream_t s);
 int launch_raa_binary(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
-                      int32_t* out, hipStream_t s);
+                      int32_t* out, hipStream_t s, int variant);
+bool raa_full_row_eligible(const zp_code* c, uint32_t D);
+
+// Commit core (api.hip): allocate a hint, and enqueue encode + Merkle on a
+// stream without synchronising (the root stays in the hint's last layer).
+int hint_alloc(const zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uint32_t batch,
+               hipStream_t s, zp_hint** out);
+int commit_enqueue(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
+                   uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
+                   bool allow_overlap);
+int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
+                        uint64_t num_rows, void* out, hipStream_t s);
+inline bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
+int check_types(const zp_code* c, int eval_type, uint32_t D);
+uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D);
+uint64_t eval_bytes_of(int eval_type);
 int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
                      uint8_t* leaves, hipStream_t s);
 uint64_t leaf_chunks(uint64_t nrows, uint32_t elem);

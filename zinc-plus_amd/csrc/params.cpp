@@ -43,7 +43,6 @@ uint64_t residue(int64_t v) {
   return (uint64_t)(m < 0 ? m + (int64_t)kP : m);
 }
 
-bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
 
 }  // namespace
 

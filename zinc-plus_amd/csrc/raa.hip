@@ -30,11 +30,6 @@ constexpr int kThreads = 512;
 
 }  // namespace
 
-// 8 planes per workgroup at N = 8192 (one workgroup per CU) instead of 4
-// (two per CU) in the plane-group kernel.
-int g_raa_wide = 1;
-// Forces the plane-group kernel where the full-row kernel would run.
-int g_raa_planes = 0;
 
 namespace {
 
@@ -247,9 +242,7 @@ int launch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t
                hipStream_t s) {
   const uint32_t N = (uint32_t)cd->N;
   const size_t lds = (size_t)N * 4 * NW + (size_t)kThreads * 2 * NW * 4;
-  const void* fn = (const void*)raa_binary_kernel<NW, S, kFlip>;
-  if (lds > 64 * 1024)
-    ZP_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  ZP_TRY((grant_lds<raa_binary_kernel<NW, S, kFlip>>(lds)));
   const uint32_t npg = (D + 2 * NW - 1) / (2 * NW);
   // Blocks are grouped 8 rows at a time: block = 8*(rowgroup*npg + pg) + r8.
   const uint64_t max_groups = std::max<uint64_t>(1, 0x7fffffffull / (8ull * npg));
@@ -368,9 +361,7 @@ int launch_raa_row(const zp_code* cd, const uint64_t* evals, uint64_t rows, int3
                    hipStream_t s) {
   const uint32_t N = (uint32_t)cd->N;
   const size_t lds = raa_row_lds(D, N);
-  if (lds > 64 * 1024)
-    ZP_HIP(hipFuncSetAttribute((const void*)raa_row_kernel<D>,
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+  ZP_TRY(grant_lds<raa_row_kernel<D>>(lds));
   for (uint64_t r0 = 0; r0 < rows; r0 += 0x7fffffffull) {
     const uint32_t nr = (uint32_t)std::min<uint64_t>(0x7fffffffull, rows - r0);
     hipLaunchKernelGGL((raa_row_kernel<D>), dim3(nr), dim3(kRowThreads), lds, s, evals, out,
@@ -382,15 +373,15 @@ int launch_raa_row(const zp_code* cd, const uint64_t* evals, uint64_t rows, int3
 
 template <bool kFlip>
 int dispatch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
-                 int32_t* out, hipStream_t s) {
+                 int32_t* out, hipStream_t s, bool wide) {
   // Planes per workgroup: the 16-bit first-scan array (N x planes x 2 B)
   // and the scan scratch must fit in 160 KiB of LDS.
   switch (cd->N) {
     case 32768: return launch_raa<1, 64, kFlip>(cd, evals, rows, D, out, s);
     case 16384: return launch_raa<2, 32, kFlip>(cd, evals, rows, D, out, s);
     case 8192:
-      return g_raa_wide ? launch_raa<4, 16, kFlip>(cd, evals, rows, D, out, s)
-                        : launch_raa<2, 16, kFlip>(cd, evals, rows, D, out, s);
+      return wide ? launch_raa<4, 16, kFlip>(cd, evals, rows, D, out, s)
+                  : launch_raa<2, 16, kFlip>(cd, evals, rows, D, out, s);
     case 4096: return launch_raa<4, 8, kFlip>(cd, evals, rows, D, out, s);
     case 2048: return launch_raa<4, 4, kFlip>(cd, evals, rows, D, out, s);
     case 1024: return launch_raa<4, 2, kFlip>(cd, evals, rows, D, out, s);
@@ -400,21 +391,26 @@ int dispatch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32
 
 }  // namespace
 
+bool raa_full_row_eligible(const zp_code* cd, uint32_t D) {
+  // Plain code, 32 or 64 planes, every stream a whole number of positions,
+  // and the bit-vectors within LDS.
+  return cd->kind == ZP_CODE_RAA && (D == 64 || D == 32) && cd->N >= 1024 &&
+         raa_row_lds(D, (uint32_t)cd->N) <= 150 * 1024;
+}
+
+// variant: 0 full-row kernel where eligible, 1 plane groups of 8 (at
+// N = 8192), 2 plane groups of 4 (at N = 8192).
 int launch_raa_binary(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
-                      int32_t* out, hipStream_t s) {
+                      int32_t* out, hipStream_t s, int variant) {
   if (rows == 0) return ZP_OK;
   if (cd->N > (1u << 15))
     return fail(ZP_ERR_INVALID_PARAM, "RAA codeword length above 32768 is not supported");
-  const uint32_t N = (uint32_t)cd->N;
-  // The full-row kernel: plain code, 32 or 64 planes, every stream a whole
-  // number of positions, and the bit-vectors within LDS.
-  if (cd->kind == ZP_CODE_RAA && !g_raa_planes && (D == 64 || D == 32) && N >= 1024 &&
-      raa_row_lds(D, N) <= 150 * 1024) {
+  if (variant == 0 && raa_full_row_eligible(cd, D))
     return D == 64 ? launch_raa_row<64>(cd, evals, rows, out, s)
                    : launch_raa_row<32>(cd, evals, rows, out, s);
-  }
-  return cd->kind == ZP_CODE_RAA_SIGNFLIP ? dispatch_raa<true>(cd, evals, rows, D, out, s)
-                                          : dispatch_raa<false>(cd, evals, rows, D, out, s);
+  const bool wide = variant != 2;
+  return cd->kind == ZP_CODE_RAA_SIGNFLIP ? dispatch_raa<true>(cd, evals, rows, D, out, s, wide)
+                                          : dispatch_raa<false>(cd, evals, rows, D, out, s, wide);
 }
 
 }  // namespace zpi

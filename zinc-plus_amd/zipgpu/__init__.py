@@ -15,6 +15,10 @@ reference:
 * ``ZipPlus.combined_row`` -- the combined-row block of ``prove_f``,
   zip-plus/src/pcs/phase_prove.rs:153-244
 * ``MerkleTree`` root / ``prove`` / ``MerkleProof`` -- zip-plus/src/merkle.rs
+* ``BatchCommit`` -- many independent ``ZipPlus::commit`` calls (as the
+  reference's callers issue them, protocol/src/prover.rs:710-726) run as one
+  device plan, optionally one HIP graph; ``LinearCode.autotune`` picks the
+  fastest kernel variant for a shape.  Neither has a reference counterpart.
 
 Device memory and streams come from torch (ROCm build); every computation
 runs in libzipgpu.so.  ``ZipError`` subclasses stand in for the reference's
@@ -31,7 +35,7 @@ __all__ = [
     "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory",
     "ZipTypes", "BinPolyTypes", "IntTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
-    "merkle_root", "lib", "set_option",
+    "merkle_root", "lib", "set_option", "BatchCommit",
 ]
 
 
@@ -69,7 +73,11 @@ def lib():
 
 
 def set_option(name, value):
-    """zp_set_option: "iprs_encoder" 0 = matrix-core encoder (default), 1 = VALU."""
+    """zp_set_option: library-wide kernel-variant overrides, -1 = automatic.
+
+    "iprs_encoder" (0 matrix-core, 1 VALU), "raa_kernel" (0 full-row,
+    1 / 2 plane groups), "commit_overlap" (1 overlap encode and hash, 0 not).
+    """
     _check(L.load().zp_set_option(name.encode(), int(value)))
 
 
@@ -179,6 +187,19 @@ class LinearCode:
                                         _stream(stream)))
         return out
 
+    def autotune(self, types, num_rows, batch=1, stream=None):
+        """zp_autotune: time this code's kernel variants at a shape, keep the fastest."""
+        _check(self._lib.zp_autotune(self._h, types.code, types.degree_plus_one, num_rows, batch,
+                                     _stream(stream)))
+        return self.tuning()
+
+    def tuning(self):
+        """The tuned variants (-1 = not tuned), numbered as in set_option."""
+        v = [ctypes.c_int32() for _ in range(3)]
+        _check(self._lib.zp_code_tuning(self._h, *[ctypes.byref(x) for x in v]))
+        return {"iprs_encoder": v[0].value, "raa_kernel": v[1].value,
+                "commit_overlap": v[2].value}
+
     def encode(self, row, types):
         """LinearCode::encode of one row."""
         if row.numel() != self.row_len():
@@ -259,7 +280,8 @@ class MerkleProof:
 class ZipPlusHint:
     """ZipPlusHint kept on the device: codewords plus every Merkle layer."""
 
-    def __init__(self, handle, types, code, num_rows):
+    def __init__(self, handle, types, code, num_rows, owner=None):
+        self._owner = owner  # a BatchCommit whose pool holds the buffers
         self._h = handle
         self._lib = L.load()
         self.types = types
@@ -405,3 +427,69 @@ class ZipPlus:
                                         _dev_ptr(coeffs.contiguous()), _dev_ptr(out),
                                         _stream(stream)))
         return out
+
+
+class BatchCommit:
+    """A plan of independent commitments run together (zp_batch_*).
+
+    ``items`` is a list of ``(pp, types)`` or ``(pp, types, batch)``: each
+    item is ``ZipPlus(types).commit(pp, polys)`` for ``batch`` polys (default
+    1, i.e. commit_single).  ``graph=True`` replays one HIP graph of the whole
+    batch from the second commit on; ``autotune=True`` tunes each distinct
+    code once.  Hints returned by ``hint(i)`` are valid until the next
+    ``commit`` of the plan.
+    """
+
+    def __init__(self, items, graph=True, autotune=False, n_streams=0):
+        self._h = None
+        self._lib = L.load()
+        self.items = []
+        descs = (L.CommitDesc * len(items))()
+        for i, it in enumerate(items):
+            pp, types = it[0], it[1]
+            batch = it[2] if len(it) > 2 else 1
+            self.items.append((pp, types, batch))
+            descs[i] = L.CommitDesc(pp.linear_code._h.value, types.code, types.degree_plus_one,
+                                    pp.num_rows, batch)
+        flags = (L.ZP_BATCH_GRAPH if graph else 0) | (L.ZP_BATCH_AUTOTUNE if autotune else 0)
+        h = ctypes.c_void_p()
+        _check(self._lib.zp_batch_create(descs, len(items), flags, n_streams, ctypes.byref(h)))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.zp_batch_destroy(self._h)
+            self._h = None
+
+    def _check_input(self, i, t):
+        pp, types, batch = self.items[i]
+        n = batch * pp.num_rows * pp.linear_code.row_len()
+        if t.dtype != types.eval_dtype or t.numel() != n:
+            raise ShapeError(L.ZP_ERR_SHAPE, f"batch item {i}: expected {n} evals of "
+                                             f"{types.eval_dtype}")
+        return _dev_ptr(t).value
+
+    def commit(self, evals, stream=None):
+        """Commit every item; evals[i] holds item i's polys. -> [ZipPlusCommitment]."""
+        if len(evals) != len(self.items):
+            raise ShapeError(L.ZP_ERR_SHAPE, "one input per batch item")
+        ptrs = (ctypes.c_void_p * len(evals))(*[self._check_input(i, t)
+                                                for i, t in enumerate(evals)])
+        roots = (ctypes.c_uint8 * (32 * len(evals)))()
+        _check(self._lib.zp_batch_commit(self._h, ptrs, roots, _stream(stream)))
+        raw = bytes(roots)
+        return [ZipPlusCommitment(raw[32 * i:32 * i + 32], self.items[i][2])
+                for i in range(len(self.items))]
+
+    def hint(self, i):
+        pp, types, _ = self.items[i]
+        h = self._lib.zp_batch_hint(self._h, i)
+        if not h:
+            _check(L.ZP_ERR_INVALID_PARAM)
+        return ZipPlusHint(ctypes.c_void_p(h), types, pp.linear_code, pp.num_rows, owner=self)
+
+    def info(self):
+        hb, sb, g = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint32()
+        _check(self._lib.zp_batch_info(self._h, ctypes.byref(hb), ctypes.byref(sb),
+                                       ctypes.byref(g)))
+        return {"hint_bytes": hb.value, "scratch_bytes": sb.value, "graph_ready": bool(g.value)}

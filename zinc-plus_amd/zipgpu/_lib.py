@@ -24,11 +24,23 @@ ZP_CODE_RAA_SIGNFLIP = 2
 ZP_EVAL_I32 = 0
 ZP_EVAL_BINPOLY = 1
 
+ZP_BATCH_GRAPH = 1
+ZP_BATCH_AUTOTUNE = 2
+
 _c_void_p = ctypes.c_void_p
 _u32 = ctypes.c_uint32
 _u64 = ctypes.c_uint64
 _i32 = ctypes.c_int32
 _int = ctypes.c_int
+
+
+class CommitDesc(ctypes.Structure):
+    """zp_commit_desc."""
+
+    _fields_ = [("code", ctypes.c_void_p), ("eval_type", ctypes.c_int32),
+                ("degree_plus_one", ctypes.c_uint32), ("num_rows", ctypes.c_uint64),
+                ("batch", ctypes.c_uint32)]
+
 
 # name -> (restype, argtypes); mirrors include/zipgpu.h one for one.
 SIGNATURES = {
@@ -55,6 +67,15 @@ SIGNATURES = {
     "zp_merkle_prove": (_int, [_c_void_p, _u64, _c_void_p, ctypes.POINTER(_u32), _c_void_p]),
     "zp_hint_open_column": (_int, [_c_void_p, _u64, _c_void_p, _c_void_p]),
     "zp_hint_free": (None, [_c_void_p]),
+    "zp_batch_create": (_int, [_c_void_p, _u32, _u32, _u32, ctypes.POINTER(_c_void_p)]),
+    "zp_batch_commit": (_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p]),
+    "zp_batch_hint": (_c_void_p, [_c_void_p, _u32]),
+    "zp_batch_info": (_int, [_c_void_p, ctypes.POINTER(_u64), ctypes.POINTER(_u64),
+                             ctypes.POINTER(_u32)]),
+    "zp_batch_destroy": (None, [_c_void_p]),
+    "zp_autotune": (_int, [_c_void_p, _int, _u32, _u64, _u32, _c_void_p]),
+    "zp_code_tuning": (_int, [_c_void_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
+                              ctypes.POINTER(_i32)]),
     "zp_combined_row": (_int, [_u32, _u32, _u64, _u64, _u32, _c_void_p, _c_void_p, _c_void_p,
                                _c_void_p, _c_void_p]),
 }

@@ -43,3 +43,41 @@ def i128_words(seed, n):
     """n i128 values uniform over all 2^128 bit patterns, as int64 [n, 2]
     little-endian words."""
     return splitmix64(seed, 2 * n).view(np.int64).reshape(n, 2)
+
+
+def row_split(kind, n, max_iprs_row=1 << 14):
+    """Row length of a commitment of n evals.
+
+    RAA: isqrt(n).next_power_of_two() (zip-plus/benches/zip_plus_benches.rs:84).
+    IPRS: min(n, 2^14), the largest row the F65537 pNTT takes at rate 1/4
+    (rows of 2^15 and above give codewords of 2^17 > 65537 points,
+    code/iprs/pntt/radix8/params.rs:59-103).
+    """
+    if kind == "iprs":
+        return min(n, max_iprs_row)
+    r = int(np.sqrt(n))
+    while r * r > n:
+        r -= 1
+    while (r + 1) * (r + 1) <= n:
+        r += 1
+    p = 1
+    while p < r:
+        p <<= 1
+    return p
+
+
+def request_mix(seed, count, log2_lo, log2_hi):
+    """`count` commit requests: log2 n uniform over log2_lo..log2_hi, D in
+    {32, 64} and code in {IPRS, RAA} each with probability 1/2, from one
+    SplitMix64 stream.  -> list of dict(kind, n, D, row_len, seed)."""
+    w = splitmix64(seed, 3 * count).reshape(count, 3)
+    span = log2_hi - log2_lo + 1
+    out = []
+    for i in range(count):
+        log2n = log2_lo + int(w[i, 0] % np.uint64(span))
+        D = 32 if int(w[i, 1] >> np.uint64(63)) == 0 else 64
+        kind = "iprs" if int(w[i, 2] >> np.uint64(63)) == 0 else "raa"
+        n = 1 << log2n
+        out.append({"kind": kind, "n": n, "D": D, "row_len": row_split(kind, n),
+                    "seed": (seed * 1000003 + i) & 0xFFFFFFFF})
+    return out
