@@ -7,13 +7,20 @@ Cw = [i64; 32] (1 GiB of codewords), BLAKE3 column Merkle tree over 65536
 leaves of 16 KiB.  One step = one ZipPlus::commit (encode + Merkle) of the
 resident input; value = evaluations committed per second.
 
-Also reported: the dominant kernel's roofline (IPRS encode, timed with HIP
-events on its own stream) and the CPU oracle on a bounded sample of the
-same workload.
+Also reported on the same line:
+  roofline      the dominant kernel (the matrix-core IPRS encoder) timed with
+                HIP events on the stream it runs on; traffic from the
+                committed rocprofv3 counter summary when present
+  cpu_baseline  the CPU oracle committing a bounded sample of the workload
+  workloads     plain Zip 2^16 i32 commit; RAA 2^22 BinaryPoly<64> encode,
+                commit and combined row; 64 mixed commits as one batch plan
+                (HIP graph) against the plain per-poly path; both IPRS
+                encoders at the headline shape.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -22,6 +29,7 @@ for _p in (ROOT, os.path.join(ROOT, "zinc-plus_amd")):
     if _p not in sys.path:
         sys.path.insert(0, _p)
 
+METRIC = "Zip+ commit evals/s at 2^20 deg-32 IPRS; batched vs plain; p99 latency"
 N_EVALS = 1 << 20
 ROW_LEN = 1 << 14
 D = 32
@@ -30,10 +38,23 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of IPRS encode per evaluation: the u64 input lane plus
 # REP * D * 8 bytes of [i64; D] codeword.
 ENCODE_BYTES_PER_EVAL = 8 + REP * D * 8
+ENCODE_KERNEL = "iprs_mfma_kernel"
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "iprs_mfma_kernel_commit_2p20_d32_pmc.txt")
 
 
 def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def traffic_per_launch():
+    """HBM bytes per encoder launch from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, each scaled by the copy-kernel calibration)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            m = re.search(r"^traffic_bytes_per_launch:\s*([0-9]+)", f.read(), re.M)
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def cpu_baseline(seconds=12.0):
@@ -62,19 +83,145 @@ def cpu_baseline(seconds=12.0):
                       f"IPRS code, BLAKE3 Merkle over 8-row columns), {reps} reps in {el:.1f} s"}
 
 
+def wall_ms(torch, fn, steps, warmup=1):
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def event_ms(torch, fn, stream, reps, warmup=2):
+    """Average duration of fn's device work on `stream`, by HIP events."""
+    with torch.cuda.stream(stream):
+        for _ in range(warmup):
+            fn(stream)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            fn(stream)
+        e1.record(stream)
+    stream.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def leg_plain_zip(torch, zipgpu, synth, dev):
+    """Plain Zip commit: 2^16 i32 evals, IPRS row 2^14 x 4 rows."""
+    n = 1 << 16
+    x = torch.from_numpy(synth.i32_evals(1, n)).to(dev)
+    code = zipgpu.IprsCode.new_with_optimal_depth(1 << 14)
+    zp = zipgpu.ZipPlus(zipgpu.IntTypes())
+    pp = zp.setup(n, code)
+
+    def step():
+        h, _ = zp.commit(pp, [x])
+        h.free()
+    ms = wall_ms(torch, step, 20, 3)
+    return {"workload": "Zip commit 2^16 i32, IPRS row 2^14 x 4, Cw i128", "commit_ms": ms,
+            "evals_per_s": n / ms * 1e3}
+
+
+def leg_raa(torch, zipgpu, synth, dev):
+    """Zip+ RAA: 2^22 BinaryPoly<64>, row 2^11 x 2^11 rows, plus combined row."""
+    n, row_len, d = 1 << 22, 1 << 11, 64
+    x = torch.from_numpy(synth.binary_polys(3, n, d)).to(dev)
+    code = zipgpu.RaaCode(row_len)
+    types = zipgpu.BinPolyTypes(d)
+    zp = zipgpu.ZipPlus(types)
+    pp = zp.setup(n, code)
+    out = torch.empty((pp.num_rows, code.codeword_len(), d), dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream()
+    enc = event_ms(torch, lambda s: code.encode_rows(x, pp.num_rows, types, out=out, stream=s),
+                   stream, 5)
+    del out
+
+    def step():
+        h, _ = zp.commit(pp, [x])
+        h.free()
+    ms = wall_ms(torch, step, 5, 1)
+    al = torch.from_numpy(synth.i128_words(5, d)).to(dev)
+    co = torch.from_numpy(synth.i128_words(6, pp.num_rows)).to(dev)
+    comb = wall_ms(torch, lambda: zp.combined_row(pp, [x], al, co), 10, 2)
+    enc_gbs = (8 + REP * d * 4) * n / (enc * 1e-3) / 1e9
+    return {"workload": "Zip+ RAA commit 2^22 BinaryPoly<64>, row 2^11 x 2^11, Cw [i32;64]",
+            "encode_ms": enc, "encode_gbs_algorithmic": enc_gbs,
+            "encode_hbm_frac": enc_gbs / HBM_PEAK_GBS, "commit_ms": ms,
+            "evals_per_s": n / ms * 1e3, "combined_row_ms": comb}
+
+
+def leg_batched(torch, zipgpu, synth, dev):
+    """64 commits, log2 n uniform in 12..20, D 32/64, IPRS/RAA: one batch plan
+    (HIP graph, autotuned variants) against one zp_commit per poly."""
+    reqs = synth.request_mix(4, 64, 12, 20)
+    total = sum(r["n"] for r in reqs)
+    codes, items, xs = {}, [], []
+    for r in reqs:
+        key = (r["kind"], r["row_len"])
+        if key not in codes:
+            codes[key] = (zipgpu.IprsCode.new_with_optimal_depth(r["row_len"])
+                          if r["kind"] == "iprs" else zipgpu.RaaCode(r["row_len"]))
+        types = zipgpu.BinPolyTypes(r["D"])
+        items.append((zipgpu.ZipPlus(types).setup(r["n"], codes[key]), types))
+        xs.append(torch.from_numpy(synth.binary_polys(r["seed"], r["n"], r["D"])).to(dev))
+
+    def plain():
+        hs = [zipgpu.ZipPlus(t).commit(pp, [x])[0] for (pp, t), x in zip(items, xs)]
+        for h in hs:
+            h.free()
+    plain_ms = wall_ms(torch, plain, 3, 1)
+    plan = zipgpu.BatchCommit(items, graph=True, autotune=True, n_streams=8)
+    roots = [c.root for c in plan.commit(xs)]
+    plan_ms = wall_ms(torch, lambda: plan.commit(xs), 5, 1)
+    tuned_plain_ms = wall_ms(torch, plain, 3, 1)
+    # The plan's roots equal the plain path's.
+    same = roots == [zipgpu.ZipPlus(t).commit(pp, [x])[1].root
+                     for (pp, t), x in zip(items, xs)]
+    info = plan.info()
+    del plan
+    return {"workload": "64 commits, log2 n ~ U{12..20}, D 32/64, IPRS (row min(n,2^14)) / "
+                        "RAA (row isqrt(n) rounded up to 2^k), seeded mix 4",
+            "total_evals": total, "plain_ms": plain_ms, "plain_evals_per_s": total / plain_ms * 1e3,
+            "plain_autotuned_ms": tuned_plain_ms, "batched_graph_ms": plan_ms,
+            "batched_evals_per_s": total / plan_ms * 1e3, "batched_vs_plain": plain_ms / plan_ms,
+            "roots_equal_plain": same, "hint_bytes": info["hint_bytes"],
+            "scratch_bytes": info["scratch_bytes"]}
+
+
+def copy_calibration():
+    """Plain 1 GiB device copies, for scaling the FETCH_SIZE / WRITE_SIZE
+    counters (run under rocprofv3 --pmc)."""
+    import torch
+    a = torch.empty(1 << 30, dtype=torch.uint8, device="cuda:0").fill_(1)
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print("copy calibration: 3 copies of 1073741824 bytes", flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-legs", action="store_true", help="skip the other workloads")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run plain 1 GiB copies (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (got --gpus {args.gpus}); the multi-GPU layout "
-              "is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (got --gpus {args.gpus}): this build runs on a "
+              "single MI355X, and the row/column sharding over several GPUs is a design, not "
+              "a measured path", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
-    import numpy as np
+    import numpy as np  # noqa: F401
     import torch
 
     import zipgpu
@@ -91,7 +238,7 @@ def main():
     log(f"workload: n=2^20 D={D} IPRS row_len={ROW_LEN} rows={pp.num_rows} "
         f"depth={code.depth} N={code.codeword_len()}")
 
-    # Whole-commit throughput.
+    # Whole-commit throughput (one step = one commit of the resident input).
     for _ in range(args.warmup):
         h, c = zp.commit(pp, [x])
         h.free()
@@ -106,27 +253,26 @@ def main():
     value = N_EVALS * args.steps / el
     log(f"commit: {ms:.3f} ms/step, {value:.4g} evals/s, root {c.root.hex()[:16]}")
 
-    # Dominant kernel: IPRS encode (one launch over 64 rows) timed with HIP
-    # events on the stream it is launched on.
+    # Dominant kernel: the IPRS encoder (one launch over 64 rows).
     stream = torch.cuda.Stream()
     out = torch.empty((pp.num_rows, code.codeword_len(), D), dtype=torch.int64, device=dev)
-    with torch.cuda.stream(stream):
-        for _ in range(3):
-            code.encode_rows(x, pp.num_rows, types, out=out, stream=stream)
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        k = max(5, args.steps)
-        e0.record(stream)
-        for _ in range(k):
-            code.encode_rows(x, pp.num_rows, types, out=out, stream=stream)
-        e1.record(stream)
-    stream.synchronize()
-    enc_ms = e0.elapsed_time(e1) / k
+
+    def enc(s):
+        code.encode_rows(x, pp.num_rows, types, out=out, stream=s)
+    enc_ms = event_ms(torch, enc, stream, max(5, args.steps))
+    zipgpu.set_option("iprs_encoder", 1)
+    try:
+        valu_ms = event_ms(torch, enc, stream, 5)
+    finally:
+        zipgpu.set_option("iprs_encoder", -1)
+    del out
     enc_bytes = ENCODE_BYTES_PER_EVAL * N_EVALS
     achieved = enc_bytes / (enc_ms * 1e-3) / 1e9
-    log(f"encode kernel: {enc_ms:.3f} ms/launch, {achieved:.1f} GB/s algorithmic")
+    log(f"encode kernel: {enc_ms:.3f} ms/launch, {achieved:.1f} GB/s algorithmic "
+        f"(VALU encoder {valu_ms:.3f} ms)")
 
     line = {
-        "metric": "Zip+ commit evals/s at 2^20 deg-32 IPRS",
+        "metric": METRIC,
         "value": value,
         "unit": "evals/s",
         "n_gpus": 1,
@@ -143,11 +289,21 @@ def main():
                                "depth 2, Cw [i64;32], BLAKE3 column Merkle (65536 leaves)",
                    "n_evals": N_EVALS, "row_len": ROW_LEN, "num_rows": pp.num_rows,
                    "degree_plus_one": D, "code": "IPRS", "rep": REP},
-        "roofline": {"bound": "hbm", "kernel": "iprs_binary_kernel",
+        "roofline": {"bound": "hbm", "kernel": ENCODE_KERNEL,
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "launch_ms": enc_ms, "bytes_per_launch": enc_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(),
+                     "launch_ms": enc_ms, "bytes_per_launch": enc_bytes,
+                     "valu_encoder_launch_ms": valu_ms},
     }
+    if not args.no_legs:
+        legs = {}
+        for name, fn in (("plain_zip_2p16_i32", leg_plain_zip), ("raa_2p22_d64", leg_raa),
+                         ("batched_64_mixed", leg_batched)):
+            log(f"leg {name} ...")
+            legs[name] = fn(torch, zipgpu, synth, dev)
+            log(f"  {legs[name]}")
+            torch.cuda.empty_cache()
+        line["workloads"] = legs
     if not args.no_cpu:
         log("cpu baseline ...")
         line["cpu_baseline"] = cpu_baseline()
