@@ -41,7 +41,8 @@ typedef enum {
   ZP_ERR_OVERFLOW = 2,      /* the CHECK "should not overflow" panics */
   ZP_ERR_SHAPE = 3,         /* assert! panics on shapes */
   ZP_ERR_DEVICE = 4,        /* HIP runtime failure */
-  ZP_ERR_OOM = 5            /* device allocation failed / admission refused */
+  ZP_ERR_OOM = 5,           /* device allocation failed / admission refused */
+  ZP_ERR_TIMEOUT = 6        /* zp_server_poll: no event within the timeout */
 } zp_status;
 
 typedef enum { ZP_CODE_IPRS = 0, ZP_CODE_RAA = 1, ZP_CODE_RAA_SIGNFLIP = 2 } zp_code_kind;
@@ -181,6 +182,63 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t degree_plus_one, uint64_t
 /* The tuned variants (-1 = not tuned), numbered as in zp_set_option. */
 int zp_code_tuning(const zp_code* code, int32_t* iprs_encoder, int32_t* raa_kernel,
                    int32_t* commit_overlap);
+
+/* ---- Commit/open service ------------------------------------------------
+ * Continuous batching on one device (no reference counterpart; it serves
+ * the reference's per-request sequence ZipPlus::commit (phase_commit.rs:
+ * 61-140), then on the challenge the combined row (phase_prove.rs:153-244)
+ * and the column openings with Merkle paths (phase_prove.rs:247-304,
+ * merkle.rs:105-120)).  `slots` worker streams run requests side by side; a
+ * free slot takes the oldest pending opening, else the oldest pending commit
+ * whose HBM footprint (codewords, Merkle layers, evaluations, scratch) fits
+ * the budget.  Codewords, layers and evaluations stay in HBM from the commit
+ * to the end of the opening.  Results arrive as events (zp_server_poll). */
+
+typedef struct zp_server zp_server;
+
+typedef struct {
+  uint64_t hbm_budget; /* bytes the server may hold for requests (0 = 3/4 of free memory) */
+  uint32_t slots;      /* requests on the device at once, 1..16 (0 = 4) */
+} zp_server_config;
+
+typedef struct {
+  uint64_t request;
+  int32_t kind;   /* 0 commit finished (root valid), 1 opening finished */
+  int32_t status; /* zp_status of the job; a failed job ends the request */
+  uint8_t root[32];
+  double queued_us;  /* arrival (submission) to start on a slot */
+  double service_us; /* start to results on the host */
+  double latency_us; /* arrival to results on the host */
+  char message[128]; /* zp_last_error of a failed job */
+} zp_server_event;
+
+int zp_server_create(const zp_server_config* cfg, zp_server** out);
+/* Queue one commit of desc->batch polys.  Exactly one of h_evals (host,
+ * copied to the device by the server) and d_evals (device; must stay valid
+ * until the request ends) is given.  Returns at once with the request id. */
+int zp_server_commit(zp_server* server, const zp_commit_desc* desc, const void* h_evals,
+                     const void* d_evals, uint64_t* request);
+/* Host buffer sizes of an opening (combined row, columns, Merkle paths). */
+int zp_server_open_sizes(zp_server* server, uint64_t request, uint32_t comb_limbs,
+                         uint32_t n_cols, uint64_t* comb_bytes, uint64_t* columns_bytes,
+                         uint64_t* siblings_bytes);
+/* Queue the opening of a committed request: the combined row in
+ * Int<comb_limbs> (0 = none) for h_alphas (batch x D i128) and h_coeffs
+ * (num_rows i128), and the n_cols (<= 256) columns h_cols with their paths.
+ * h_comb: row_len x comb_limbs u64; h_columns: per column, batch*num_rows
+ * elements; h_siblings: per column, log2(N) x 32 bytes bottom-up.  The
+ * buffers must stay valid until the opening's event; the request's device
+ * memory is released after it. */
+int zp_server_open(zp_server* server, uint64_t request, uint32_t comb_limbs,
+                   const int64_t* h_alphas, const int64_t* h_coeffs, const uint64_t* h_cols,
+                   uint32_t n_cols, uint64_t* h_comb, uint8_t* h_columns, uint8_t* h_siblings);
+/* Next finished job; timeout_us < 0 waits without limit.  ZP_ERR_TIMEOUT
+ * when none finished in time. */
+int zp_server_poll(zp_server* server, int64_t timeout_us, zp_server_event* event);
+int zp_server_stats(zp_server* server, uint64_t* hbm_in_use, uint64_t* hbm_peak,
+                    uint64_t* budget, uint64_t* evals_committed, uint64_t* pending);
+/* Stops the workers after their current job and releases every request. */
+void zp_server_destroy(zp_server* server);
 
 /* ---- Prover combined row (zip-plus/src/pcs/phase_prove.rs:153-244) ----- */
 

@@ -204,6 +204,7 @@ const char* zp_status_string(int status) {
     case ZP_ERR_SHAPE: return "shape";
     case ZP_ERR_DEVICE: return "device";
     case ZP_ERR_OOM: return "out of memory";
+    case ZP_ERR_TIMEOUT: return "timeout";
   }
   return "unknown";
 }

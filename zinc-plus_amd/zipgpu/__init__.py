@@ -35,7 +35,7 @@ __all__ = [
     "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory",
     "ZipTypes", "BinPolyTypes", "IntTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
-    "merkle_root", "lib", "set_option", "BatchCommit",
+    "merkle_root", "lib", "set_option", "BatchCommit", "CommitServer",
 ]
 
 
@@ -493,3 +493,116 @@ class BatchCommit:
         _check(self._lib.zp_batch_info(self._h, ctypes.byref(hb), ctypes.byref(sb),
                                        ctypes.byref(g)))
         return {"hint_bytes": hb.value, "scratch_bytes": sb.value, "graph_ready": bool(g.value)}
+
+
+class CommitServer:
+    """Continuous-batching commit/open service on one device (zp_server_*).
+
+    ``commit(pp, types, evals)`` queues ZipPlus::commit of the polys in
+    ``evals`` (a host numpy array, copied to the device by the server, or a
+    device tensor) and returns a request id at once.  When the request's
+    commit event has arrived, ``open(req, ...)`` queues its opening: the
+    combined row for (alphas, coeffs) and the listed columns with their
+    Merkle paths.  ``poll`` returns finished jobs as dicts.  Inputs and output
+    buffers are kept alive here until their job's event has been polled.
+    """
+
+    def __init__(self, hbm_budget=0, slots=4):
+        self._h = None
+        self._lib = L.load()
+        cfg = L.ServerConfig(hbm_budget, slots)
+        h = ctypes.c_void_p()
+        _check(self._lib.zp_server_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._keep = {}     # request -> (pp, types, batch, input)
+        self._opening = {}  # request -> output buffers
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.zp_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def commit(self, pp, types, evals, batch=1):
+        n = batch * pp.num_rows * pp.linear_code.row_len()
+        desc = L.CommitDesc(pp.linear_code._h.value, types.code, types.degree_plus_one,
+                            pp.num_rows, batch)
+        if isinstance(evals, torch.Tensor):
+            if evals.dtype != types.eval_dtype or evals.numel() != n:
+                raise ShapeError(L.ZP_ERR_SHAPE, f"expected {n} evals of {types.eval_dtype}")
+            h_ptr, d_ptr = None, _dev_ptr(evals)
+        else:
+            want = np.int64 if types.eval_type == "binpoly" else np.int32
+            evals = np.ascontiguousarray(evals)
+            if evals.dtype.itemsize != np.dtype(want).itemsize or evals.size != n:
+                raise ShapeError(L.ZP_ERR_SHAPE, f"expected {n} evals of {np.dtype(want)}")
+            h_ptr, d_ptr = evals.ctypes.data_as(ctypes.c_void_p), None
+        rid = ctypes.c_uint64()
+        _check(self._lib.zp_server_commit(self._h, ctypes.byref(desc), h_ptr, d_ptr,
+                                          ctypes.byref(rid)))
+        self._keep[rid.value] = (pp, types, batch, evals)
+        return rid.value
+
+    def open(self, request, alphas=None, coeffs=None, cols=(), comb_limbs=None):
+        """alphas: int64 [batch, D, 2] (i128 words), coeffs: int64 [num_rows, 2];
+        cols: column indices.  comb_limbs defaults to the types' CombR limbs."""
+        if request not in self._keep:
+            raise InvalidPcsParam(L.ZP_ERR_INVALID_PARAM, "unknown request")
+        pp, types, batch, _ = self._keep[request]
+        limbs = 0 if alphas is None else (comb_limbs or types.comb_limbs)
+        cols = np.ascontiguousarray(np.asarray(cols, dtype=np.uint64))
+        sizes = [ctypes.c_uint64() for _ in range(3)]
+        _check(self._lib.zp_server_open_sizes(self._h, request, limbs, len(cols),
+                                              *[ctypes.byref(x) for x in sizes]))
+        comb = np.zeros(max(1, sizes[0].value // 8), dtype=np.uint64)
+        columns = np.zeros(max(1, sizes[1].value), dtype=np.uint8)
+        sib = np.zeros(max(1, sizes[2].value), dtype=np.uint8)
+        al = co = None
+        if limbs:
+            al = np.ascontiguousarray(alphas, dtype=np.int64)
+            co = np.ascontiguousarray(coeffs, dtype=np.int64)
+            if al.size != 2 * batch * types.degree_plus_one or co.size != 2 * pp.num_rows:
+                raise ShapeError(L.ZP_ERR_SHAPE, "alphas/coeffs have the wrong length")
+        vp = ctypes.c_void_p
+        _check(self._lib.zp_server_open(
+            self._h, request, limbs, al.ctypes.data_as(vp) if limbs else None,
+            co.ctypes.data_as(vp) if limbs else None, cols.ctypes.data_as(vp), len(cols),
+            comb.ctypes.data_as(vp), columns.ctypes.data_as(vp), sib.ctypes.data_as(vp)))
+        self._opening[request] = (limbs, cols, comb, columns, sib, al, co,
+                                  sizes[1].value, sizes[2].value)
+
+    def poll(self, timeout=None):
+        """Next finished job as a dict, or None after `timeout` seconds."""
+        ev = L.ServerEvent()
+        t = -1 if timeout is None else int(timeout * 1e6)
+        rc = self._lib.zp_server_poll(self._h, t, ctypes.byref(ev))
+        if rc == L.ZP_ERR_TIMEOUT:
+            return None
+        _check(rc)
+        out = {"request": ev.request, "kind": "commit" if ev.kind == 0 else "open",
+               "status": ev.status, "queued_us": ev.queued_us, "service_us": ev.service_us,
+               "latency_us": ev.latency_us,
+               "message": ev.message.decode(errors="replace")}
+        if ev.kind == 0:
+            out["root"] = bytes(ev.root)
+            if ev.status != L.ZP_OK:
+                self._keep.pop(ev.request, None)
+        else:
+            pp, types, batch, _ = self._keep.pop(ev.request)
+            limbs, cols, comb, columns, sib, _, _, cb, sb = self._opening.pop(ev.request)
+            row_len = pp.linear_code.row_len()
+            out["comb"] = comb[:row_len * limbs].reshape(row_len, limbs) if limbs else None
+            per = cb // max(1, len(cols))
+            out["columns"] = [columns[k * per:(k + 1) * per].tobytes() for k in range(len(cols))]
+            h = sb // max(1, len(cols)) // 32
+            out["siblings"] = [[sib[(k * h + l) * 32:(k * h + l + 1) * 32].tobytes()
+                                for l in range(h)] for k in range(len(cols))]
+        return out
+
+    def stats(self):
+        v = [ctypes.c_uint64() for _ in range(5)]
+        _check(self._lib.zp_server_stats(self._h, *[ctypes.byref(x) for x in v]))
+        keys = ["hbm_in_use", "hbm_peak", "budget", "evals_committed", "pending"]
+        return dict(zip(keys, (x.value for x in v)))

@@ -16,6 +16,7 @@ ZP_ERR_OVERFLOW = 2
 ZP_ERR_SHAPE = 3
 ZP_ERR_DEVICE = 4
 ZP_ERR_OOM = 5
+ZP_ERR_TIMEOUT = 6
 
 ZP_CODE_IPRS = 0
 ZP_CODE_RAA = 1
@@ -40,6 +41,21 @@ class CommitDesc(ctypes.Structure):
     _fields_ = [("code", ctypes.c_void_p), ("eval_type", ctypes.c_int32),
                 ("degree_plus_one", ctypes.c_uint32), ("num_rows", ctypes.c_uint64),
                 ("batch", ctypes.c_uint32)]
+
+
+class ServerConfig(ctypes.Structure):
+    """zp_server_config."""
+
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("slots", ctypes.c_uint32)]
+
+
+class ServerEvent(ctypes.Structure):
+    """zp_server_event."""
+
+    _fields_ = [("request", ctypes.c_uint64), ("kind", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("root", ctypes.c_uint8 * 32),
+                ("queued_us", ctypes.c_double), ("service_us", ctypes.c_double),
+                ("latency_us", ctypes.c_double), ("message", ctypes.c_char * 128)]
 
 
 # name -> (restype, argtypes); mirrors include/zipgpu.h one for one.
@@ -76,6 +92,16 @@ SIGNATURES = {
     "zp_autotune": (_int, [_c_void_p, _int, _u32, _u64, _u32, _c_void_p]),
     "zp_code_tuning": (_int, [_c_void_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
                               ctypes.POINTER(_i32)]),
+    "zp_server_create": (_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
+    "zp_server_commit": (_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p,
+                                ctypes.POINTER(_u64)]),
+    "zp_server_open_sizes": (_int, [_c_void_p, _u64, _u32, _u32, ctypes.POINTER(_u64),
+                                    ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
+    "zp_server_open": (_int, [_c_void_p, _u64, _u32, _c_void_p, _c_void_p, _c_void_p, _u32,
+                              _c_void_p, _c_void_p, _c_void_p]),
+    "zp_server_poll": (_int, [_c_void_p, ctypes.c_int64, _c_void_p]),
+    "zp_server_stats": (_int, [_c_void_p] + [ctypes.POINTER(_u64)] * 5),
+    "zp_server_destroy": (None, [_c_void_p]),
     "zp_combined_row": (_int, [_u32, _u32, _u64, _u64, _u32, _c_void_p, _c_void_p, _c_void_p,
                                _c_void_p, _c_void_p]),
 }
