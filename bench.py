@@ -14,8 +14,9 @@ Also reported on the same line:
   cpu_baseline  the CPU oracle committing a bounded sample of the workload
   workloads     plain Zip 2^16 i32 commit; RAA 2^22 BinaryPoly<64> encode,
                 commit and combined row; 64 mixed commits as one batch plan
-                (HIP graph) against the plain per-poly path; both IPRS
-                encoders at the headline shape.
+                (HIP graph) against the plain per-poly path; the commit/open
+                service under Poisson arrivals (p50/p99 latency) against one
+                request at a time; both IPRS encoders at the headline shape.
 """
 import argparse
 import json
@@ -190,6 +191,97 @@ def leg_batched(torch, zipgpu, synth, dev):
             "scratch_bytes": info["scratch_bytes"]}
 
 
+def served_requests(zipgpu, synth, seed, count, lo, hi):
+    import numpy as np
+    codes, out = {}, []
+    for r in synth.request_mix(seed, count, lo, hi):
+        key = (r["kind"], r["row_len"])
+        if key not in codes:
+            codes[key] = (zipgpu.IprsCode.new_with_optimal_depth(r["row_len"])
+                          if r["kind"] == "iprs" else zipgpu.RaaCode(r["row_len"]))
+        types = zipgpu.BinPolyTypes(r["D"], 5)
+        pp = zipgpu.ZipPlus(types).setup(r["n"], codes[key])
+        N = codes[key].codeword_len()
+        out.append({"pp": pp, "types": types, "n": r["n"],
+                    "x": synth.binary_polys(r["seed"], r["n"], r["D"]),
+                    "al": synth.i128_words(r["seed"] + 1, r["D"]),
+                    "co": synth.i128_words(r["seed"] + 2, pp.num_rows),
+                    "cols": (synth.splitmix64(r["seed"] + 3, 147) % np.uint64(N))})
+    return out, codes
+
+
+def serve(zipgpu, reqs, slots, arrivals, open_delays):
+    """Run the requests through a CommitServer: commit i arrives at
+    arrivals[i] (seconds from the start), its opening open_delays[i] after
+    its root is back.  -> (wall seconds, commit latencies us, open latencies us)."""
+    import heapq
+    srv = zipgpu.CommitServer(slots=slots)
+    t0 = time.perf_counter()
+    due = [(a, 0, i) for i, a in enumerate(arrivals)]  # (time, kind 0 commit / 1 open, index)
+    heapq.heapify(due)
+    ids, lat_c, lat_o, left = {}, [], [], 2 * len(reqs)
+    while left:
+        now = time.perf_counter() - t0
+        while due and due[0][0] <= now:
+            _, kind, i = heapq.heappop(due)
+            q = reqs[i]
+            if kind == 0:
+                ids[srv.commit(q["pp"], q["types"], q["x"])] = i
+            else:
+                srv.open(q["rid"], q["al"], q["co"], q["cols"])
+        wait = max(0.0, due[0][0] - (time.perf_counter() - t0)) if due else 1.0
+        ev = srv.poll(min(wait, 1.0))
+        if ev is None:
+            continue
+        if ev["status"] != 0:
+            raise RuntimeError(f"served request failed: {ev}")
+        i = ids[ev["request"]]
+        if ev["kind"] == "commit":
+            lat_c.append(ev["latency_us"])
+            reqs[i]["rid"] = ev["request"]
+            reqs[i]["root"] = ev["root"]
+            heapq.heappush(due, (time.perf_counter() - t0 + open_delays[i], 1, i))
+        else:
+            lat_o.append(ev["latency_us"])
+        left -= 1
+    wall = time.perf_counter() - t0
+    peak = srv.stats()["hbm_peak"]
+    srv.close()
+    return wall, lat_c, lat_o, peak
+
+
+def leg_served(torch, zipgpu, synth, dev):
+    """Commit+open service: Poisson arrivals, log2 n uniform in 14..22, D 32/64,
+    IPRS/RAA; openings (combined row + 147 columns with Merkle paths) a
+    seeded delay after each commit."""
+    import numpy as np
+    count, rate = 32, 300.0  # requests, arrivals per second
+    reqs, _ = served_requests(zipgpu, synth, 5, count, 14, 22)
+    total = sum(q["n"] for q in reqs)
+    u = (synth.splitmix64(77, 2 * count) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    gaps = -np.log1p(-u[:count]) / rate
+    arrivals = np.cumsum(gaps) - gaps[0]
+    delays = -np.log1p(-u[count:]) * 1e-3  # openings: mean 1 ms after the root
+    serve(zipgpu, reqs[:4], 4, [0.0] * 4, [0.0] * 4)  # warm the codes and the device cache
+    wall, lc, lo, peak = serve(zipgpu, reqs, 4, arrivals, delays)
+    roots = [q["root"] for q in reqs]
+    sat4, _, _, _ = serve(zipgpu, reqs, 4, [0.0] * count, [0.0] * count)
+    sat1, _, _, _ = serve(zipgpu, reqs, 1, [0.0] * count, [0.0] * count)
+    same = roots == [q["root"] for q in reqs]
+    pct = lambda a, p: float(np.percentile(np.asarray(a), p))  # noqa: E731
+    return {"workload": f"{count} commit+open requests, Poisson {rate:.0f}/s, log2 n ~ U{{14..22}}, "
+                        "D 32/64, IPRS/RAA, host inputs; openings: combined row Int<5> + 147 "
+                        "columns with Merkle paths, exp(1 ms) after each root; 4 slots",
+            "total_evals": total, "wall_s": wall, "evals_per_s": total / wall,
+            "commit_latency_us_p50": pct(lc, 50), "commit_latency_us_p99": pct(lc, 99),
+            "open_latency_us_p50": pct(lo, 50), "open_latency_us_p99": pct(lo, 99),
+            "hbm_peak_bytes": peak,
+            "saturated_4_slots_evals_per_s": total / sat4,
+            "one_at_a_time_evals_per_s": total / sat1,
+            "served_vs_one_at_a_time": sat1 / sat4,
+            "roots_equal_across_runs": same}
+
+
 def copy_calibration():
     """Plain 1 GiB device copies, for scaling the FETCH_SIZE / WRITE_SIZE
     counters (run under rocprofv3 --pmc)."""
@@ -298,7 +390,7 @@ def main():
     if not args.no_legs:
         legs = {}
         for name, fn in (("plain_zip_2p16_i32", leg_plain_zip), ("raa_2p22_d64", leg_raa),
-                         ("batched_64_mixed", leg_batched)):
+                         ("batched_64_mixed", leg_batched), ("served_commit_open", leg_served)):
             log(f"leg {name} ...")
             legs[name] = fn(torch, zipgpu, synth, dev)
             log(f"  {legs[name]}")

@@ -139,7 +139,8 @@ def test_device_hashes(zg, dev, golden):
     for e in v["blake3"]:
         if e["len"] == 0 or e["len"] % 4:
             continue  # device elements are whole 32-bit words (every codeword type is)
-        data = np.frombuffer(bytes((31 * i + 7) % 256 for i in range(e["len"])), dtype=np.uint8).copy()
+        raw = bytes((31 * i + 7) % 256 for i in range(e["len"]))
+        data = np.frombuffer(raw, dtype=np.uint8).copy()
         # One row of one element: the root of a single-leaf tree is the leaf.
         root = zg.merkle_root(to_dev(data, dev).reshape(1, 1, -1), e["len"])
         assert root.hex() == e["hash"], e["len"]
