@@ -345,6 +345,17 @@ def main():
     value = N_EVALS * args.steps / el
     log(f"commit: {ms:.3f} ms/step, {value:.4g} evals/s, root {c.root.hex()[:16]}")
 
+    # A caller holding the evaluations in host memory: copy in, then commit.
+    pcie = {}
+    for kind, xh in (("pinned", x.cpu().pin_memory()), ("pageable", x.cpu())):
+        def step_h():
+            x.copy_(xh, non_blocking=True)
+            h, _ = zp.commit(pp, [x])
+            h.free()
+        ms_h = wall_ms(torch, step_h, 10, 2)
+        pcie[kind] = {"ms_per_commit": ms_h, "evals_per_s": N_EVALS / ms_h * 1e3}
+    log(f"host-input commit: {pcie}")
+
     # Dominant kernel: the IPRS encoder (one launch over 64 rows).
     stream = torch.cuda.Stream()
     out = torch.empty((pp.num_rows, code.codeword_len(), D), dtype=torch.int64, device=dev)
@@ -386,6 +397,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(),
                      "launch_ms": enc_ms, "bytes_per_launch": enc_bytes,
                      "valu_encoder_launch_ms": valu_ms},
+        "host_input_commit": pcie,
     }
     if not args.no_legs:
         legs = {}
