@@ -46,9 +46,21 @@ __device__ __forceinline__ int64_t dbl_to_i64(double x) {
   return __double_as_longlong(x + magic) - __double_as_longlong(magic);
 }
 
+// Stage-1 inputs y (|y| < 2^41) are split as y = 2^32 yh + yl with
+// yl in [0, 2^32), so every product and partial sum of both halves is an
+// integer below 2^50 in magnitude.  The accumulators are biased so that
+// their bit patterns are the values they hold: a double in [2^52, 2^53) has
+// ulp 1, so M + v (M = 1.5 * 2^52, |v| < 2^51) is exact and
+// bits(M + v) = bits(M) + v.  The low half starts at M, the high half at
+// M - K with K = bits(M) >> 32 (bits(M) has no set bit below 51).  Then,
+// modulo 2^64,
+//   bits(lo) + (bits(hi) << 32) = bits(M) + sl + ((sh - K) << 32) = sl + 2^32 sh,
+// and since the low word of bits(M) is zero, (bits(hi) << 32) only needs the
+// low word of bits(hi): the output is one 32-bit add on the high word.
 constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
 constexpr uint64_t kMagicBits = 0x4338000000000000ull;
-constexpr uint64_t kMagicFix = (kMagicBits << 21) + kMagicBits;
+constexpr double kMagicHi = kMagic - (double)(kMagicBits >> 32);
+static_assert((kMagicBits & 0xffffffffull) == 0, "bias must be a multiple of 2^32");
 
 // planes[((row*NC + c)*W + w)*D + p] holds the 16 bits of plane p for
 // k = 16w .. 16w+15 in "spread" order: bit i = 4v + b of the group sits at
@@ -122,13 +134,16 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
 
   // Issue every global load of phase 1 up front: the plane words of this
   // wave's chunks, the matrix slices, and the block's stage-0 twiddles.
-  const uint32_t* prow = planes + row * NC * W * D + 16 * pg + pcol;
+  // One row's plane words span NC*W*D*4 <= 256 KiB: 32-bit offsets from the
+  // (uniform) row base.
+  const char* prow = (const char*)(planes + row * NC * W * D);
+  const uint32_t poff = ((wave * W + g) * D + 16 * pg + pcol) * 4;
   uint32_t pw[kTiles][S];
 #pragma unroll
   for (uint32_t i = 0; i < kTiles; i++)
 #pragma unroll
     for (int s = 0; s < S; s++)
-      pw[i][s] = prow[((uint64_t)(wave + 4 * i) * W + 4 * s + g) * D];
+      pw[i][s] = *(const uint32_t*)(prow + (poff + (4 * i * W + 4 * s) * D * 4));
   v4i A[S][3];
 #pragma unroll
   for (int s = 0; s < S; s++)
@@ -155,7 +170,7 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
     }
 #pragma unroll
     for (int v = 0; v < 4; v++)
-      Bs[(ci * 16 + 4 * g + v) * 16 + pcol] = acc0[v] + 128 * acc1[v] + 16384 * acc2[v];
+      Bs[(ci * 16 + 4 * g + v) * 16 + pcol] = (((acc2[v] << 7) + acc1[v]) << 7) + acc0[v];
   }
   __syncthreads();
 
@@ -199,23 +214,24 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
       T1[i] = tw1[((uint64_t)cl * bd + rb * 16) * 56 + rem];
     }
     __syncthreads();
-    const double two21 = 2097152.0, inv21 = 1.0 / 2097152.0;
+    const double two32 = 4294967296.0, inv32 = 1.0 / 4294967296.0;
 #pragma unroll
     for (int cl = 0; cl < 8; cl++) {
       double yh[8], yl[8];
 #pragma unroll
       for (int j = 0; j < 8; j++) {
         const double v = y0[8 * j + cl];
-        yh[j] = __builtin_floor(v * inv21);
-        yl[j] = __builtin_fma(yh[j], -two21, v);
+        yh[j] = __builtin_floor(v * inv32);
+        yl[j] = __builtin_fma(yh[j], -two32, v);
       }
       const double* t1 = T1 + (cl * 16 + rl) * 56;
       // Sixteen independent chains: high and low halves of the 8 outputs.
       double sh[8], sl[8];
+      const double h0 = yh[0] + kMagicHi, l0 = yl[0] + kMagic;
 #pragma unroll
       for (int t = 0; t < 8; t++) {
-        sh[t] = yh[0];
-        sl[t] = yl[0];
+        sh[t] = h0;
+        sl[t] = l0;
       }
 #pragma unroll
       for (int j = 1; j < 8; j++)
@@ -228,11 +244,9 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
       uint32_t ooff = obase + (uint32_t)cl * cstride;  // output chunk 8t + cl
 #pragma unroll
       for (int t = 0; t < 8; t++) {
-        // (sh + M) and (sl + M) hold sh and sl in their low mantissa bits;
-        // out = (bits(sh+M) << 21) + bits(sl+M) - bits(M) * (2^21 + 1).
-        const uint64_t hb = (uint64_t)__double_as_longlong(sh[t] + kMagic);
-        const uint64_t lb = (uint64_t)__double_as_longlong(sl[t] + kMagic);
-        *(int64_t*)(orow + ooff) = (int64_t)((hb << 21) + lb - kMagicFix);
+        const uint32_t hlo = (uint32_t)__double_as_longlong(sh[t]);
+        const uint64_t lb = (uint64_t)__double_as_longlong(sl[t]);
+        *(uint2*)(orow + ooff) = make_uint2((uint32_t)lb, (uint32_t)(lb >> 32) + hlo);
         ooff += 8 * cstride;
       }
     }
