@@ -23,7 +23,9 @@
 // (row, 16 base rows r, 16 planes p) and all 8^d chunks: MFMA tiles are
 // (16 r) x (16 planes) for one chunk, the summed base values go to LDS, and
 // each thread then runs every stage of one (r, p) column and writes its
-// outputs; the 16 planes of one (c, r) are 128 contiguous bytes.
+// outputs; the 16 planes of one (c, r) are 128 contiguous bytes.  The 16
+// threads of one r form one 16-lane DPP row, which lets the butterflies read
+// r's twiddles from each other's registers instead of from LDS.
 #include "common.h"
 
 namespace zpi {
@@ -92,19 +94,116 @@ bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ plane
   }
 }
 
+// Radix-8 butterfly steps with the twiddles spread over each 16-lane row of
+// a wave (the lanes of a row share one base row r): lane q of the row holds
+// entries q, q+16, q+32, q+48 (< 56) of that r's 56 twiddles in tw[0..3], and
+// each FMA reads the entry it needs from lane k of its row with a
+// row_newbcast DPP source (v_fmac_f64_dpp), so no lane holds or loads more
+// than 4 of them.  Step jj (1..7) of output t uses entry (jj-1)*8 + (t*jj & 7),
+// i.e. register (jj-1)/2 at lane L(jj, t) = ((jj-1)*8 & 15) + (t*jj & 7).
+//
+// The compiler does not look inside inline asm, so each block starts with the
+// two wait states a DPP read needs after a VALU write of its source.
+template <int JJ, int T>
+constexpr int bcast_lane() {
+  return (((JJ - 1) * 8) & 15) + ((T * JJ) & 7);
+}
+
+// acc[t] += x * tw(lane L(JJ, t)) for t = 0..7, and the same for acc2 with x2.
+template <int JJ>
+__device__ __forceinline__ void bfly16(double (&acc)[8], double (&acc2)[8], double tw, double x,
+                                       double x2) {
+  asm("s_nop 1\n"
+      "v_fmac_f64_dpp %0, %16, %17 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %8, %16, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %16, %17 row_newbcast:%20 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %9, %16, %18 row_newbcast:%20 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %16, %17 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %10, %16, %18 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %16, %17 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %11, %16, %18 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %4, %16, %17 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %12, %16, %18 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %5, %16, %17 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %13, %16, %18 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %6, %16, %17 row_newbcast:%25 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %14, %16, %18 row_newbcast:%25 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %7, %16, %17 row_newbcast:%26 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %15, %16, %18 row_newbcast:%26 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]),
+        "+v"(acc[6]), "+v"(acc[7]), "+v"(acc2[0]), "+v"(acc2[1]), "+v"(acc2[2]), "+v"(acc2[3]),
+        "+v"(acc2[4]), "+v"(acc2[5]), "+v"(acc2[6]), "+v"(acc2[7])
+      : "v"(tw), "v"(x), "v"(x2), "i"(bcast_lane<JJ, 0>()), "i"(bcast_lane<JJ, 1>()),
+        "i"(bcast_lane<JJ, 2>()), "i"(bcast_lane<JJ, 3>()), "i"(bcast_lane<JJ, 4>()),
+        "i"(bcast_lane<JJ, 5>()), "i"(bcast_lane<JJ, 6>()), "i"(bcast_lane<JJ, 7>()));
+}
+
+// Steps JJ and JJ+1 (same twiddle register) on one set of accumulators:
+// acc[t] += x * tw(L(JJ, t)), then acc[t] += x2 * tw(L(JJ+1, t)).
+template <int JJ>
+__device__ __forceinline__ void bfly_pair(double (&acc)[8], double tw, double x, double x2) {
+  asm("s_nop 1\n"
+      "v_fmac_f64_dpp %0, %8, %9 row_newbcast:%11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %8, %9 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %8, %9 row_newbcast:%13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %8, %9 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %4, %8, %9 row_newbcast:%15 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %5, %8, %9 row_newbcast:%16 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %6, %8, %9 row_newbcast:%17 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %7, %8, %9 row_newbcast:%18 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %0, %8, %10 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %8, %10 row_newbcast:%20 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %8, %10 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %8, %10 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %4, %8, %10 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %5, %8, %10 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %6, %8, %10 row_newbcast:%25 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %7, %8, %10 row_newbcast:%26 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]),
+        "+v"(acc[6]), "+v"(acc[7])
+      : "v"(tw), "v"(x), "v"(x2), "i"(bcast_lane<JJ, 0>()), "i"(bcast_lane<JJ, 1>()),
+        "i"(bcast_lane<JJ, 2>()), "i"(bcast_lane<JJ, 3>()), "i"(bcast_lane<JJ, 4>()),
+        "i"(bcast_lane<JJ, 5>()), "i"(bcast_lane<JJ, 6>()), "i"(bcast_lane<JJ, 7>()),
+        "i"(bcast_lane<JJ + 1, 0>()), "i"(bcast_lane<JJ + 1, 1>()), "i"(bcast_lane<JJ + 1, 2>()),
+        "i"(bcast_lane<JJ + 1, 3>()), "i"(bcast_lane<JJ + 1, 4>()), "i"(bcast_lane<JJ + 1, 5>()),
+        "i"(bcast_lane<JJ + 1, 6>()), "i"(bcast_lane<JJ + 1, 7>()));
+}
+
+// Step 7 alone.
+__device__ __forceinline__ void bfly_last(double (&acc)[8], double tw, double x) {
+  asm("s_nop 1\n"
+      "v_fmac_f64_dpp %0, %8, %9 row_newbcast:%10 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %8, %9 row_newbcast:%11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %8, %9 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %8, %9 row_newbcast:%13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %4, %8, %9 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %5, %8, %9 row_newbcast:%15 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %6, %8, %9 row_newbcast:%16 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %7, %8, %9 row_newbcast:%17 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]),
+        "+v"(acc[6]), "+v"(acc[7])
+      : "v"(tw), "v"(x), "i"(bcast_lane<7, 0>()), "i"(bcast_lane<7, 1>()),
+        "i"(bcast_lane<7, 2>()), "i"(bcast_lane<7, 3>()), "i"(bcast_lane<7, 4>()),
+        "i"(bcast_lane<7, 5>()), "i"(bcast_lane<7, 6>()), "i"(bcast_lane<7, 7>()));
+}
+
+// The 56 twiddles of one r (a table row at `row`), spread over the 16 lanes
+// of this lane's row: entry 16m + q on lane q (the address is clamped for the
+// lanes past entry 55 of m = 3, whose value is never read).
+__device__ __forceinline__ void load_spread(double (&tw)[4], const double* row, uint32_t q) {
+#pragma unroll
+  for (int m = 0; m < 4; m++) tw[m] = row[min(16u * m + q, 55u)];
+}
+
 template <int NC, int S>
 __global__ void __launch_bounds__(256, 2)
 iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ afrag,
                  const double* __restrict__ tw0, const double* __restrict__ tw1,
                  int64_t* __restrict__ out, uint32_t bd, uint32_t D, uint32_t nrb, uint32_t npg,
                  uint64_t row0) {
-  // Bs holds the base sums [c][r_local][p] until stage 0 has read them; the
-  // same bytes then hold this block's stage-1 twiddles [c_lo][r_local][56].
+  // Base sums [c][r_local][p], from the matrix cores to the stage threads.
   constexpr uint32_t kBsWords = NC * 16 * 16;
-  constexpr uint32_t kTw1Doubles = NC == 64 ? 8 * 16 * 56 : 0;
-  static_assert(kTw1Doubles * 2 <= kBsWords, "stage-1 twiddles must fit in the base area");
   __shared__ int32_t Bs[kBsWords];
-  __shared__ double T0[16 * 56];
   constexpr uint32_t W = 4 * S;
   constexpr uint32_t kTiles = NC / 4;  // chunks per wave
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -132,8 +231,9 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
     }
   }
 
-  // Issue every global load of phase 1 up front: the plane words of this
-  // wave's chunks, the matrix slices, and the block's stage-0 twiddles.
+  // Issue every global load up front: the plane words of this wave's chunks,
+  // the matrix slices and this thread's share of its r's stage-0 twiddles
+  // (phase 2 has rl = tid / 16 and p = tid % 16).
   // One row's plane words span NC*W*D*4 <= 256 KiB: 32-bit offsets from the
   // (uniform) row base.
   const char* prow = (const char*)(planes + row * NC * W * D);
@@ -149,7 +249,11 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   for (int s = 0; s < S; s++)
 #pragma unroll
     for (int q = 0; q < 3; q++) A[s][q] = afrag[(((uint64_t)rb * S + s) * 3 + q) * 64 + lane];
-  for (uint32_t i = tid; i < 16 * 56; i += 256) T0[i] = tw0[(uint64_t)rb * 16 * 56 + i];
+  double t0r[4];
+  load_spread(t0r, tw0 + (uint64_t)(rb * 16 + (tid >> 4)) * 56, tid & 15);
+  // Keep the loads here: left to the scheduler they sink to their first
+  // uses and their latency is exposed there.
+  __builtin_amdgcn_sched_barrier(0);
 
   // Phase 1: base layer on the matrix cores.
 #pragma unroll
@@ -183,22 +287,30 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   char* orow = (char*)(out + (row0 + row) * N * D);
   const uint32_t obase = (r * D + 16 * pg + p) * 8;  // byte offset of chunk 0
   const uint32_t cstride = bd * D * 8;               // byte stride between chunks
-  const double* t0 = T0 + rl * 56;
 
+  // Stage 1's first twiddles are loaded before stage 0, which hides them.
+  double t1n[4];
+  if constexpr (NC == 64) {
+    load_spread(t1n, tw1 + (uint64_t)r * 56, p);
+    __builtin_amdgcn_sched_barrier(0);
+  }
+
+  // Stage 0: one radix-8 butterfly per block of 8 chunks.
   double y0[NC];
 #pragma unroll
   for (int j = 0; j < NC / 8; j++) {
     double b[8];
 #pragma unroll
     for (int jj = 0; jj < 8; jj++) b[jj] = (double)Bs[((8 * j + jj) * 16 + rl) * 16 + p];
-    // Eight independent accumulation chains (one per output t).
+    double acc[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) y0[8 * j + t] = b[0];
+    for (int t = 0; t < 8; t++) acc[t] = b[0];
+    bfly_pair<1>(acc, t0r[0], b[1], b[2]);
+    bfly_pair<3>(acc, t0r[1], b[3], b[4]);
+    bfly_pair<5>(acc, t0r[2], b[5], b[6]);
+    bfly_last(acc, t0r[3], b[7]);
 #pragma unroll
-    for (int jj = 1; jj < 8; jj++)
-#pragma unroll
-      for (int t = 0; t < 8; t++)
-        y0[8 * j + t] = __builtin_fma(b[jj], t0[(jj - 1) * 8 + ((t * jj) & 7)], y0[8 * j + t]);
+    for (int t = 0; t < 8; t++) y0[8 * j + t] = acc[t];
   }
 
   if constexpr (NC == 8) {
@@ -206,17 +318,20 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
     for (int c = 0; c < 8; c++)
       *(int64_t*)(orow + (obase + (uint32_t)c * cstride)) = dbl_to_i64(y0[c]);
   } else {
-    // Stage-1 twiddles of rows r_local < 16 and c_lo < 8 into the base area.
-    __syncthreads();
-    double* T1 = (double*)Bs;
-    for (uint32_t i = tid; i < kTw1Doubles; i += 256) {
-      const uint32_t cl = i / (16 * 56), rem = i % (16 * 56);
-      T1[i] = tw1[((uint64_t)cl * bd + rb * 16) * 56 + rem];
-    }
-    __syncthreads();
+    // Stage 1: output chunk 8t + cl from the inputs 8j + cl.  The next
+    // step's twiddles are loaded before this step's stores: the memory
+    // counter covers stores too, so a load issued after them would wait for
+    // them as well.
     const double two32 = 4294967296.0, inv32 = 1.0 / 4294967296.0;
 #pragma unroll
     for (int cl = 0; cl < 8; cl++) {
+      double t1r[4];
+#pragma unroll
+      for (int m = 0; m < 4; m++) t1r[m] = t1n[m];
+      if (cl < 7) {
+        load_spread(t1n, tw1 + ((uint64_t)(cl + 1) * bd + r) * 56, p);
+        __builtin_amdgcn_sched_barrier(0);
+      }
       double yh[8], yl[8];
 #pragma unroll
       for (int j = 0; j < 8; j++) {
@@ -224,7 +339,6 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
         yh[j] = __builtin_floor(v * inv32);
         yl[j] = __builtin_fma(yh[j], -two32, v);
       }
-      const double* t1 = T1 + (cl * 16 + rl) * 56;
       // Sixteen independent chains: high and low halves of the 8 outputs.
       double sh[8], sl[8];
       const double h0 = yh[0] + kMagicHi, l0 = yl[0] + kMagic;
@@ -233,14 +347,13 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
         sh[t] = h0;
         sl[t] = l0;
       }
-#pragma unroll
-      for (int j = 1; j < 8; j++)
-#pragma unroll
-        for (int t = 0; t < 8; t++) {
-          const double tv = t1[(j - 1) * 8 + ((t * j) & 7)];
-          sh[t] = __builtin_fma(yh[j], tv, sh[t]);
-          sl[t] = __builtin_fma(yl[j], tv, sl[t]);
-        }
+      bfly16<1>(sh, sl, t1r[0], yh[1], yl[1]);
+      bfly16<2>(sh, sl, t1r[0], yh[2], yl[2]);
+      bfly16<3>(sh, sl, t1r[1], yh[3], yl[3]);
+      bfly16<4>(sh, sl, t1r[1], yh[4], yl[4]);
+      bfly16<5>(sh, sl, t1r[2], yh[5], yl[5]);
+      bfly16<6>(sh, sl, t1r[2], yh[6], yl[6]);
+      bfly16<7>(sh, sl, t1r[3], yh[7], yl[7]);
       uint32_t ooff = obase + (uint32_t)cl * cstride;  // output chunk 8t + cl
 #pragma unroll
       for (int t = 0; t < 8; t++) {
