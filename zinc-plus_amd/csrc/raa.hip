@@ -270,7 +270,9 @@ int launch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t
 // measured slower.  Phase 1 puts
 // positions in lanes and turns each plane's bits into 64-bit words with one
 // ballot per plane; phase 2 puts planes in lanes, so each output position
-// (4*D bytes) is written by one store instruction.
+// (4*D bytes) is written by one store instruction.  perm2 is copied to LDS
+// before any store: the memory counter also counts stores, so a global load
+// of perm2 between the stores would wait for all of them.
 constexpr uint32_t kRowThreads = 512;
 constexpr uint32_t kRowWaves = kRowThreads / 64;
 
@@ -288,19 +290,42 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   uint64_t* bits = lds64;                                  // [D][WP]
   uint16_t* cp = (uint16_t*)(lds64 + (size_t)D * WP);      // [D][CP]
   uint32_t* part = (uint32_t*)(lds64 + (size_t)D * WP + ((size_t)D * CP + 3) / 4);
+  uint32_t* p2 = part + 2 * kRowThreads;                   // [N] perm2
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t row = row0 + blockIdx.x;
   const uint64_t* x = evals + row * row_len;
   const uint32_t rmask = row_len - 1;
 
-  // Phase 1a: y1 bit-vectors, 64 positions per wave step.
-  for (uint32_t blk = wave; blk < W; blk += kRowWaves) {
-    const uint32_t src = perm1[blk * 64 + lane];
-    const uint64_t xv = x[src & rmask];
-#pragma unroll 8
-    for (int p = 0; p < D; p++) {
-      const uint64_t m = __ballot((xv >> p) & 1ull);
-      if (lane == 0) bits[p * WP + blk] = m;
+  for (uint32_t i = tid; i < N; i += kRowThreads) p2[i] = perm2[i];
+  // Phase 1a: y1 bit-vectors, 64 positions per wave step; the permutation
+  // and gather loads of up to kPre steps are issued together.
+  constexpr uint32_t kPre = 4;
+  for (uint32_t b0 = wave; b0 < W; b0 += kRowWaves * kPre) {
+    uint32_t src[kPre];
+    uint64_t xv[kPre];
+#pragma unroll
+    for (uint32_t k = 0; k < kPre; k++) {
+      const uint32_t blk = b0 + k * kRowWaves;
+      src[k] = blk < W ? perm1[blk * 64 + lane] : 0u;
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kPre; k++) xv[k] = x[src[k] & rmask];
+#pragma unroll
+    for (uint32_t k = 0; k < kPre; k++) {
+      const uint32_t blk = b0 + k * kRowWaves;
+      if (blk >= W) break;
+      // Lane q collects plane q's word from the q-th ballot.
+      const uint32_t lo = (uint32_t)xv[k], hi = (uint32_t)(xv[k] >> 32);
+      uint32_t wlo = 0, whi = 0;
+#pragma unroll
+      for (int q = 0; q < D; q++) {
+        const uint64_t m = __ballot((((q < 32) ? (lo >> q) : (hi >> (q - 32))) & 1u) != 0u);
+        if (lane == (uint32_t)q) {
+          wlo = (uint32_t)m;
+          whi = (uint32_t)(m >> 32);
+        }
+      }
+      if (lane < (uint32_t)D) bits[lane * WP + blk] = ((uint64_t)whi << 32) | wlo;
     }
   }
   __syncthreads();
@@ -325,35 +350,61 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   }
   __syncthreads();
 
-  // Phase 2: second accumulate, planes in lanes.  Stream s covers positions
-  // [s*L, (s+1)*L); pass 1 sums, the streams exchange carries, pass 2 emits.
+  // Phase 2: second accumulate, planes in lanes, in one pass of N/(NS*K)
+  // steps.  In each step stream s takes K consecutive positions, prefixes
+  // them in registers, and the streams exchange their totals through LDS
+  // (double-buffered, one barrier per step) to get their carries.
   const uint32_t sub = lane / D, p = lane % D;
   const uint32_t s = wave * SUB + sub;
-  const uint32_t L = N / NS;
-  const uint32_t i0 = s * L, i1 = i0 + L;
+  constexpr uint32_t K = 64 / SUB;
+  const uint32_t steps = N / (NS * K);
   const uint64_t* bp = bits + p * WP;
   const uint16_t* cpp = cp + p * CP;
   auto rank = [&](uint32_t m) -> uint32_t {
     const uint64_t mask = (2ull << (m & 63)) - 1ull;  // ones(m % 64 + 1)
     return cpp[m >> 6] + __popcll(bp[m >> 6] & mask);
   };
-  uint32_t sum = 0;
-  for (uint32_t i = i0; i < i1; i++) sum += rank(perm2[i]);
-  __syncthreads();
-  part[s * D + p] = sum;
-  __syncthreads();
-  uint32_t run = 0;
-  for (uint32_t sg = 0; sg < s; sg++) run += part[sg * D + p];
-  int32_t* o = out + (row * N + i0) * D + p;
-  for (uint32_t i = i0; i < i1; i++) {
-    run += rank(perm2[i]);
-    o[(uint64_t)(i - i0) * D] = (int32_t)run;
+  int32_t* orow = out + row * N * D + p;
+  uint32_t run = 0;  // every position of the earlier steps
+  for (uint32_t st = 0; st < steps; st++) {
+    const uint32_t base = (st * NS + s) * K;
+    uint32_t v[K];
+    // The lookups of 16 positions are independent: all are issued first.
+#pragma unroll
+    for (uint32_t k0 = 0; k0 < K; k0 += 16) {
+      uint32_t m[16];
+#pragma unroll
+      for (uint32_t k = 0; k < 16; k++) m[k] = p2[base + k0 + k];
+#pragma unroll
+      for (uint32_t k = 0; k < 16; k++) v[k0 + k] = rank(m[k]);
+    }
+#pragma unroll
+    for (uint32_t k = 1; k < K; k++) v[k] += v[k - 1];
+    uint32_t* tot = part + (st & 1) * NS * D;
+    tot[s * D + p] = v[K - 1];
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < NS; q++) {
+      const uint32_t t = tot[q * D + p];
+      all += t;
+      before += q < s ? t : 0u;
+    }
+    const uint32_t c = run + before;
+#pragma unroll
+    for (uint32_t k = 0; k < K; k++) orow[(uint64_t)(base + k) * D] = (int32_t)(c + v[k]);
+    run += all;
   }
 }
 
+// bits, checkpoints, the exchange area (phase-1b partial counts, then two
+// buffers of per-stream totals: 2 * 8 * 64 words), and perm2.
+constexpr uint32_t kRowPartWords = 2 * kRowThreads;
+
 size_t raa_row_lds(uint32_t D, uint32_t N) {
   const size_t WP = N / 64 + 1, CP = N / 64 + 1;
-  return (size_t)D * WP * 8 + (((size_t)D * CP + 3) / 4) * 8 + (size_t)kRowThreads * 4;
+  return (size_t)D * WP * 8 + (((size_t)D * CP + 3) / 4) * 8 + (size_t)kRowPartWords * 4 +
+         (size_t)N * 4;
 }
 
 template <int D>
