@@ -71,7 +71,7 @@ const char* zp_last_error(void);
  *                   1 = plane groups (8 planes per workgroup at codeword
  *                       length 8192), 2 = plane groups (4 planes at 8192).
  *   "commit_overlap": 1 = hash finished row groups on a second stream while
- *                   later rows encode (default), 0 = encode, then hash.
+ *                   later rows encode, 0 = encode, then hash (default).
  * Returns ZP_ERR_INVALID_PARAM for an unknown name or value. */
 int zp_set_option(const char* name, int64_t value);
 

@@ -107,7 +107,7 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
   const uint64_t rpc = (elem <= 1024 && 1024 % elem == 0) ? 1024 / elem : 0;  // rows per chunk
   const uint64_t nchunks = leaf_chunks(rows_all, (uint32_t)elem);
   const uint32_t kGroups = 4;
-  const bool pipelined = allow_overlap && pick(g_force_overlap, code->tune_overlap, 1) == 1 &&
+  const bool pipelined = allow_overlap && pick(g_force_overlap, code->tune_overlap, 0) == 1 &&
                          batch == 1 && rpc && num_rows % (rpc * kGroups) == 0 &&
                          nchunks >= kGroups;
   if (!pipelined) {

@@ -197,7 +197,7 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
   auto commit_all = [&] {
     return commit_enqueue(code, eval_type, D, evals, num_rows, batch, h, s, true);
   };
-  tune(&code->tune_overlap, {1, 0}, commit_all);
+  tune(&code->tune_overlap, {0, 1}, commit_all);
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
   dev_free(evals, s);

@@ -273,8 +273,13 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
       acc2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][2], bf, acc2, 0, 0, 0);
     }
 #pragma unroll
-    for (int v = 0; v < 4; v++)
-      Bs[(ci * 16 + 4 * g + v) * 16 + pcol] = (((acc2[v] << 7) + acc1[v]) << 7) + acc0[v];
+    for (int v = 0; v < 4; v++) {
+      // Two shift-adds per value: the empty asm keeps the compiler from
+      // re-associating into two shifts and a three-way add.
+      int32_t t = (acc2[v] << 7) + acc1[v];
+      asm("" : "+v"(t));
+      Bs[(ci * 16 + 4 * g + v) * 16 + pcol] = (t << 7) + acc0[v];
+    }
   }
   __syncthreads();
 

@@ -194,6 +194,69 @@ cv_level_kernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uin
   }
 }
 
+// The whole chunk tree of G columns per workgroup, in LDS: the same
+// level-by-level left pairing as cv_level_kernel, with the levels in two
+// ping-pong areas (level 0 in A, level 1 in B, level 2 in A, ...) and one
+// barrier per level; the root merge is written to the leaf array.
+constexpr uint32_t kTreeLdsWords = 48 * 1024 / 4;
+
+__global__ void __launch_bounds__(256)
+cv_tree_kernel(const uint32_t* __restrict__ cvs, uint64_t width, uint32_t nchunks, uint32_t G,
+               uint8_t* __restrict__ leaves) {
+  __shared__ uint4 lds[kTreeLdsWords / 4];
+  uint4* bufs[2] = {lds, lds + (size_t)nchunks * G * 2};
+  const uint64_t j0 = (uint64_t)blockIdx.x * G;
+  const uint32_t gh = (uint32_t)min((uint64_t)G, width - j0);
+  for (uint32_t i = threadIdx.x; i < nchunks * gh * 2; i += blockDim.x) {
+    const uint32_t ci = i / (gh * 2), rem = i % (gh * 2);
+    bufs[0][(ci * G) * 2 + rem] = ((const uint4*)cvs)[((uint64_t)ci * width + j0) * 2 + rem];
+  }
+  __syncthreads();
+  uint32_t n = nchunks, cur = 0;
+  while (n > 1) {
+    const uint32_t n_out = (n + 1) / 2;
+    const uint4* in = bufs[cur];
+    uint4* out = bufs[cur ^ 1];
+    for (uint32_t t = threadIdx.x; t < n_out * gh; t += blockDim.x) {
+      const uint32_t pi = t / gh, g = t % gh;
+      const uint4* a = in + ((2 * pi) * G + g) * 2;
+      uint32_t r[8];
+      const uint4 a0 = a[0], a1 = a[1];
+      r[0] = a0.x; r[1] = a0.y; r[2] = a0.z; r[3] = a0.w;
+      r[4] = a1.x; r[5] = a1.y; r[6] = a1.z; r[7] = a1.w;
+      if (2 * pi + 1 < n) {
+        uint32_t l[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++) l[i] = r[i];
+        const uint4* b = in + ((2 * pi + 1) * G + g) * 2;
+        const uint4 b0 = b[0], b1 = b[1];
+        r[0] = b0.x; r[1] = b0.y; r[2] = b0.z; r[3] = b0.w;
+        r[4] = b1.x; r[5] = b1.y; r[6] = b1.z; r[7] = b1.w;
+        parent_cv(l, r, n_out == 1);
+      }
+      if (n_out == 1) {
+        uint32_t* dst = (uint32_t*)(leaves + (j0 + g) * 32);
+#pragma unroll
+        for (int i = 0; i < 8; i++) dst[i] = r[i];
+      } else {
+        uint4* o = out + (pi * G + g) * 2;
+        o[0] = make_uint4(r[0], r[1], r[2], r[3]);
+        o[1] = make_uint4(r[4], r[5], r[6], r[7]);
+      }
+    }
+    __syncthreads();
+    n = n_out;
+    cur ^= 1;
+  }
+}
+
+// Columns per workgroup of cv_tree_kernel (0: the tree does not fit).
+uint32_t cv_tree_columns(uint64_t nchunks) {
+  const uint64_t per_col = (nchunks + (nchunks + 1) / 2) * 8;  // words, both areas
+  const uint64_t g = kTreeLdsWords / per_col;
+  return (uint32_t)std::min<uint64_t>(g, 64);
+}
+
 // Merkle layers (merkle.rs:172-221) for n = 2^h leaves stored at layers[0..n).
 // Layer L starts at node offset 2n - 2(n >> L).  A workgroup of 256 threads
 // reads 512 nodes of layer L0 and produces up to 9 layers above them in LDS,
@@ -266,6 +329,14 @@ int launch_chunk_cvs(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32
 // the leaf; `tmp` holds ceil(nchunks/2) x width values.
 int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
                     uint8_t* leaves, hipStream_t s) {
+  if (nchunks <= 1) return ZP_OK;
+  const uint32_t G = cv_tree_columns(nchunks);
+  if (G > 0) {
+    hipLaunchKernelGGL(cv_tree_kernel, dim3((uint32_t)((width + G - 1) / G)), dim3(256), 0, s, cvs,
+                       width, (uint32_t)nchunks, G, leaves);
+    ZP_HIP(hipGetLastError());
+    return ZP_OK;
+  }
   uint32_t* buf[2] = {cvs, tmp};
   uint64_t n = nchunks;
   int cur = 0;

@@ -76,7 +76,8 @@ def set_option(name, value):
     """zp_set_option: library-wide kernel-variant overrides, -1 = automatic.
 
     "iprs_encoder" (0 matrix-core, 1 VALU), "raa_kernel" (0 full-row,
-    1 / 2 plane groups), "commit_overlap" (1 overlap encode and hash, 0 not).
+    1 / 2 plane groups), "commit_overlap" (1 overlap encode and hash, 0 not:
+    the default).
     """
     _check(L.load().zp_set_option(name.encode(), int(value)))
 
