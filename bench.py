@@ -39,7 +39,7 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes of IPRS encode per evaluation: the u64 input lane plus
 # REP * D * 8 bytes of [i64; D] codeword.
 ENCODE_BYTES_PER_EVAL = 8 + REP * D * 8
-ENCODE_KERNEL = "iprs_mfma_kernel"
+ENCODE_KERNEL = "bitplane_kernel + iprs_mfma_kernel (one encode_rows)"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "iprs_mfma_kernel_commit_2p20_d32_pmc.txt")
 
 
