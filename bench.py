@@ -153,6 +153,26 @@ def leg_raa(torch, zipgpu, synth, dev):
             "evals_per_s": n / ms * 1e3, "combined_row_ms": comb}
 
 
+def leg_encode_wide(torch, zipgpu, synth, dev):
+    """LinearCode::encode_wide of CombR = Int<5> rows: the verifier's
+    encoding of the combined row (one row), and 16 rows for throughput."""
+    L, out = 5, {}
+    for name, code in (("iprs_2p14", zipgpu.IprsCode.new_with_optimal_depth(1 << 14)),
+                       ("raa_2p11", zipgpu.RaaCode(1 << 11)),
+                       ("raa_flip_2p11", zipgpu.RaaSignFlippingCode(1 << 11))):
+        rl, N = code.row_len(), code.codeword_len()
+        res = {}
+        for rows in (1, 16):
+            x = torch.from_numpy(synth.comb_rows(8, rows * rl, L).reshape(rows, rl, L)).to(dev)
+            stream = torch.cuda.Stream()
+            ms = event_ms(torch, lambda s: code.encode_wide(x, stream=s), stream, 10)
+            res[f"rows{rows}_ms"] = ms
+            res[f"rows{rows}_gbs_algorithmic"] = rows * (rl + N) * L * 8 / (ms * 1e-3) / 1e9
+        out[name] = res
+    out["workload"] = "encode_wide Int<5>: IPRS row 2^14 (depth 2), RAA row 2^11, rate 1/4"
+    return out
+
+
 def leg_batched(torch, zipgpu, synth, dev):
     """64 commits, log2 n uniform in 12..20, D 32/64, IPRS/RAA: one batch plan
     (HIP graph, autotuned variants) against one zp_commit per poly."""
@@ -402,7 +422,8 @@ def main():
     if not args.no_legs:
         legs = {}
         for name, fn in (("plain_zip_2p16_i32", leg_plain_zip), ("raa_2p22_d64", leg_raa),
-                         ("batched_64_mixed", leg_batched), ("served_commit_open", leg_served)):
+                         ("batched_64_mixed", leg_batched), ("served_commit_open", leg_served),
+                         ("encode_wide_int5", leg_encode_wide)):
             log(f"leg {name} ...")
             legs[name] = fn(torch, zipgpu, synth, dev)
             log(f"  {legs[name]}")

@@ -102,6 +102,16 @@ uint64_t zp_cw_elem_bytes(const zp_code* code, int eval_type, uint32_t degree_pl
 int zp_encode_rows(const zp_code* code, int eval_type, uint32_t degree_plus_one,
                    const void* d_evals, uint64_t num_rows, void* d_out, zp_stream stream);
 
+/* LinearCode::encode_wide (zip-plus/src/code.rs:55; iprs.rs:148-150,
+ * raa.rs:151-153, raa_sign_flip.rs:97-99): the code applied to num_rows
+ * rows of row_len CombR = Int<comb_limbs> values (comb_limbs little-endian
+ * u64 limbs each, two's complement), arithmetic mod 2^(64*comb_limbs).
+ * d_in: num_rows x row_len x comb_limbs u64; d_out: num_rows x
+ * codeword_len x comb_limbs u64.  comb_limbs in 1..8.  Asynchronous on
+ * `stream`. */
+int zp_encode_wide(const zp_code* code, uint32_t comb_limbs, const uint64_t* d_in,
+                   uint64_t num_rows, uint64_t* d_out, zp_stream stream);
+
 /* ---- Merkle tree (zip-plus/src/merkle.rs) ----------------------------- */
 
 /* MerkleTree::new(rows).root() (merkle.rs:74-102) over a device row matrix

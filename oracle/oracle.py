@@ -242,3 +242,50 @@ def alpha_projection(evals, alphas, D=64, L=5):
     assert load().or_alpha_projection(_u64(D), _u64(L), _u64(len(ev)), _p(ev), _p(al),
                                       _p(out)) == 0
     return out
+
+
+def iprs_encode_wide(rows, row_len, depth=-1, L=5, rep=4):
+    """LinearCode::encode_wide for IPRS: uint64 [num_rows, row_len, L] limbs of
+    Int<L> -> uint64 [num_rows, N, L] (arithmetic mod 2^(64L))."""
+    x = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1, row_len, L)
+    out = np.zeros((x.shape[0], row_len * rep, L), dtype=np.uint64)
+    rc = load().or_iprs_encode_wide(_u64(row_len), _i64(depth), _u64(rep), _u64(L), _p(x),
+                                    _u64(x.shape[0]), _p(out))
+    if rc:
+        raise ValueError(f"oracle rc {rc}")
+    return out
+
+
+def raa_encode_wide(rows, row_len, L=5, rep=4, sign_flip=False):
+    """LinearCode::encode_wide for RAA / RaaSignFlipping over Int<L> limbs."""
+    x = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1, row_len, L)
+    out = np.zeros((x.shape[0], row_len * rep, L), dtype=np.uint64)
+    rc = load().or_raa_encode_wide(_u64(row_len), _u64(rep), ctypes.c_int(int(sign_flip)),
+                                   _u64(L), _p(x), _u64(x.shape[0]), _p(out))
+    if rc:
+        raise ValueError(f"oracle rc {rc}")
+    return out
+
+
+def limbs_add(a, b):
+    """Limb-wise a + b mod 2^(64L) over the last axis (uint64 limbs)."""
+    a = np.asarray(a, dtype=np.uint64)
+    b = np.asarray(b, dtype=np.uint64)
+    out = np.empty_like(a)
+    carry = np.zeros(a.shape[:-1], dtype=np.uint64)
+    for i in range(a.shape[-1]):
+        s = a[..., i] + b[..., i]
+        c1 = (s < a[..., i]).astype(np.uint64)
+        s2 = s + carry
+        c2 = (s2 < s).astype(np.uint64)
+        out[..., i] = s2
+        carry = c1 | c2
+    return out
+
+
+def sign_extend_limbs(words, L):
+    """int64 [..., k] little-endian words of a two's complement value -> uint64 [..., L]."""
+    w = np.asarray(words, dtype=np.int64)
+    ext = np.where(w[..., -1:] < 0, np.int64(-1), np.int64(0))
+    pad = np.repeat(ext, L - w.shape[-1], axis=-1)
+    return np.concatenate([w, pad], axis=-1).view(np.uint64)

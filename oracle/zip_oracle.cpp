@@ -966,3 +966,114 @@ extern "C" int or_alpha_projection(uint64_t D, uint64_t L, uint64_t n, const uin
   }
   return OR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// LinearCode::encode_wide (zip-plus/src/code.rs:55, IprsCode
+// code/iprs.rs:148-150 -> encode_inner :54-87 with In = Out = CombR;
+// RaaCode code/raa.rs:151-153 and RaaSignFlippingCode
+// code/raa_sign_flip.rs:97-99 -> encode_inner).  CombR = Int<L>; the pNTT
+// multiplies by the i64 table entries (Int<L>::mul_by_scalar(&i64),
+// utils/src/mul_by_scalar.rs:31-74) and adds, all modulo 2^(64L) (the
+// unchecked build; the checked build panics instead of wrapping, so the two
+// agree on every input that does not overflow).
+// ---------------------------------------------------------------------------
+static void big_neg(BigL& a, int L) {
+  u128 c = 1;
+  for (int i = 0; i < L; i++) {
+    c += (u128)(~a.w[i]);
+    a.w[i] = (uint64_t)c;
+    c >>= 64;
+  }
+}
+static BigL big_load(const uint64_t* p, int L) {
+  BigL a;
+  for (int i = 0; i < L; i++) a.w[i] = p[i];
+  return a;
+}
+static void big_store(uint64_t* p, const BigL& a, int L) {
+  for (int i = 0; i < L; i++) p[i] = a.w[i];
+}
+
+// One row: in [row_len][L] limbs -> out [N][L] limbs (radix8.rs:21-161).
+static void pntt_wide_row(const PnttParams& p, int L, const uint64_t* in, uint64_t* out) {
+  const uint64_t N = p.codeword_len, bd = p.base_dim, bl = p.base_len, d = p.depth;
+  const uint64_t shift = 3 * d;
+  for (uint64_t i = 0; i < N; i++) {
+    const uint64_t chunk = i >> p.base_dim_log2, row = i & (bd - 1);
+    const uint64_t o = octrev(chunk, d);
+    BigL acc = big_load(in + o * L, L);  // from_ref
+    const int64_t* mrow = &p.base_matrix[row * bl];
+    for (uint64_t col = 1; col < bl; col++) {
+      BigL t;
+      big_mul_i128(t, big_load(in + (o | (col << shift)) * L, L), (i128)mrow[col], L);
+      big_add(acc, t, L);
+    }
+    big_store(out + i * L, acc, L);
+  }
+  for (uint64_t k = 0; k < d; k++) {
+    const uint64_t sub = bd << (3 * k);
+    const int64_t* tw = p.twiddles[k].data();
+    for (uint64_t blk = 0; blk < N; blk += 8 * sub) {
+      uint64_t* chunk = out + blk * L;
+      for (uint64_t i = 0; i < sub; i++) {
+        BigL xs[8];
+        for (uint64_t j = 0; j < 8; j++) xs[j] = big_load(chunk + (j * sub + i) * L, L);
+        const int64_t* twi = tw + i * 56;
+        for (uint64_t r = 0; r < 8; r++) {
+          BigL a = xs[0];
+          for (uint64_t jm1 = 0; jm1 < 7; jm1++) {
+            BigL t;
+            big_mul_i128(t, xs[jm1 + 1], (i128)twi[jm1 * 8 + BUTTERFLY_TABLE[r][jm1]], L);
+            big_add(a, t, L);
+          }
+          big_store(chunk + (r * sub + i) * L, a, L);
+        }
+      }
+    }
+  }
+}
+
+extern "C" int or_iprs_encode_wide(uint64_t row_len, int64_t depth, uint64_t rep, uint64_t L,
+                                   const uint64_t* in, uint64_t num_rows, uint64_t* out) {
+  if (L < 1 || L > 8) return OR_ERR_INVALID_PARAM;
+  PnttParams p;
+  int rc = resolve_params(row_len, depth, rep, p);
+  if (rc) return rc;
+  parallel_for(num_rows, [&](uint64_t r) {
+    pntt_wide_row(p, (int)L, in + r * row_len * L, out + r * p.codeword_len * L);
+  });
+  return OR_OK;
+}
+
+// raa.rs:93-126 over Int<L> (sign flips: raa_sign_flip.rs:45-57).
+extern "C" int or_raa_encode_wide(uint64_t row_len, uint64_t rep, int sign_flip, uint64_t L,
+                                  const uint64_t* in, uint64_t num_rows, uint64_t* out) {
+  if (L < 1 || L > 8 || !is_pow2(rep) || !is_pow2(row_len)) return OR_ERR_INVALID_PARAM;
+  const uint64_t N = row_len * rep;
+  std::vector<uint64_t> p1(N), p2(N);
+  or_raa_perm(N, 1, p1.data());
+  or_raa_perm(N, 2, p2.data());
+  const int l = (int)L;
+  parallel_for(num_rows, [&](uint64_t r) {
+    const uint64_t* x = in + r * row_len * L;
+    uint64_t* dst = out + r * N * L;
+    std::vector<BigL> tmp(N), a(N);
+    for (uint64_t i = 0; i < N; i++) {
+      tmp[i] = big_load(x + (i % row_len) * L, l);  // repeat
+      if (sign_flip && (i & 1)) big_neg(tmp[i], l);
+    }
+    BigL acc;
+    big_from_i128(acc, 0, l);
+    for (uint64_t i = 0; i < N; i++) {  // perm1, accumulate
+      big_add(acc, tmp[p1[i]], l);
+      a[i] = acc;
+      if (sign_flip && (i & 1)) big_neg(a[i], l);
+    }
+    big_from_i128(acc, 0, l);
+    for (uint64_t i = 0; i < N; i++) {  // perm2, accumulate
+      big_add(acc, a[p2[i]], l);
+      big_store(dst + i * L, acc, l);
+    }
+  });
+  return OR_OK;
+}

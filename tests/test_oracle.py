@@ -5,6 +5,8 @@ Each test names what fixes the expected value: a known answer from the
 reference's own tests, a public test vector, or mathematics (a closed form,
 an invariant, an independent computation).
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -422,3 +424,53 @@ def test_alpha_projection_matches_definition(oracle):
                 v = big(alphas[c])
                 want += v - 2**128 if v >= 2**127 else v
         assert big(got[k]) == want % 2**320
+
+
+# ---------------------------------------------------------------------------
+# encode_wide over Int<L> (code.rs:55; iprs.rs:148-150; raa.rs:151-153)
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("row_len,depth,L", [(256, 1, 3), (512, 2, 5), (2048, 3, 8), (64, 1, 4)])
+def test_iprs_wide_equals_i32_path_on_small_inputs(oracle, row_len, depth, L):
+    # i32 rows sign-extended to Int<L>: the wide encoder must give the i128
+    # codewords of the (pinned) i32 path, sign-extended to L limbs.
+    from zipgpu import synth
+    rows = 2
+    x = synth.i32_evals(31 + row_len, rows * row_len)
+    want = oracle.iprs_encode_i32(x, rows, row_len, depth)           # [rows, N, 2]
+    wide_in = oracle.sign_extend_limbs(x.astype(np.int64)[:, None], L).reshape(rows, row_len, L)
+    got = oracle.iprs_encode_wide(wide_in, row_len, depth, L)
+    assert np.array_equal(got, oracle.sign_extend_limbs(want, L))
+
+
+@pytest.mark.parametrize("kind,L", [("iprs", 5), ("raa", 5), ("raa_flip", 3), ("iprs", 8)])
+def test_wide_linear_mod_2_64L(oracle, kind, L):
+    # Full-range limbs: encode(x + y) = encode(x) + encode(y) modulo 2^(64L).
+    from zipgpu import synth
+    row_len = 256
+    x = synth.splitmix64(5, row_len * L).reshape(1, row_len, L)
+    y = synth.splitmix64(6, row_len * L).reshape(1, row_len, L)
+    if kind == "iprs":
+        enc = lambda v: oracle.iprs_encode_wide(v, row_len, -1, L)  # noqa: E731
+    else:
+        enc = lambda v: oracle.raa_encode_wide(v, row_len, L, sign_flip=kind == "raa_flip")  # noqa: E731
+    lhs = enc(oracle.limbs_add(x, y))
+    rhs = oracle.limbs_add(enc(x), enc(y))
+    assert np.array_equal(lhs, rhs)
+    assert not np.array_equal(enc(x), enc(y))
+
+
+@pytest.mark.parametrize("flip", [False, True])
+def test_raa_wide_equals_i64_path(oracle, flip):
+    # Small integer rows: the wide RAA encoder agrees with the i64 restatement
+    # that reproduces the raa.rs:396-413 known answer.
+    row_len, L = 64, 4
+    rng = np.random.default_rng(3)
+    row = rng.integers(-1000, 1000, row_len).astype(np.int64)
+    want = np.zeros(row_len * 4, dtype=np.int64)
+    assert oracle.load().or_raa_encode_i64(ctypes.c_uint64(row_len), ctypes.c_uint64(4),
+                                          ctypes.c_int(int(flip)), oracle._p(row),
+                                          oracle._p(want)) == 0
+    got = oracle.raa_encode_wide(oracle.sign_extend_limbs(row[:, None], L)[None], row_len, L,
+                                 sign_flip=flip)
+    assert np.array_equal(got[0], oracle.sign_extend_limbs(want[:, None], L))

@@ -317,6 +317,14 @@ int zp_encode_rows(const zp_code* code, int eval_type, uint32_t D, const void* d
   return encode_rows_variant(code, eval_type, D, d_evals, num_rows, d_out, (hipStream_t)stream);
 }
 
+int zp_encode_wide(const zp_code* code, uint32_t L, const uint64_t* d_in, uint64_t num_rows,
+                   uint64_t* d_out, zp_stream stream) {
+  if (!code) return fail(ZP_ERR_INVALID_PARAM, "null code");
+  if (L < 1 || L > 8) return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 1..8");
+  if (num_rows && (!d_in || !d_out)) return fail(ZP_ERR_INVALID_PARAM, "null buffer");
+  return launch_encode_wide(code, L, d_in, num_rows, d_out, (hipStream_t)stream);
+}
+
 int zp_merkle_root(const void* d_rows, uint64_t num_rows, uint64_t width, uint32_t elem,
                    uint8_t h_root[32], zp_stream stream) {
   hipStream_t s = (hipStream_t)stream;

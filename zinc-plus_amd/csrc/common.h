@@ -133,6 +133,9 @@ int commit_enqueue(const zp_code* code, int eval_type, uint32_t D, const void* d
                    bool allow_overlap);
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
                         uint64_t num_rows, void* out, hipStream_t s);
+// wide.hip: LinearCode::encode_wide over Int<L>, L in 1..8.
+int launch_encode_wide(const zp_code* c, uint32_t L, const uint64_t* in, uint64_t rows,
+                       uint64_t* out, hipStream_t s);
 inline bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
 int check_types(const zp_code* c, int eval_type, uint32_t D);
 uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D);

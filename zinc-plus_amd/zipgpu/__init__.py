@@ -188,6 +188,27 @@ class LinearCode:
                                         _stream(stream)))
         return out
 
+    def encode_wide(self, rows, stream=None):
+        """LinearCode::encode_wide (code.rs:55) of rows of CombR = Int<L>.
+
+        rows: device int64 [num_rows, row_len, L] or [row_len, L] (u64 limbs,
+        little-endian, two's complement); returns [.., codeword_len, L].
+        """
+        if rows.dtype != torch.int64:
+            raise ValueError("CombR rows must be int64 limbs")
+        single = rows.dim() == 2
+        r = rows.unsqueeze(0) if single else rows
+        if r.dim() != 3 or r.shape[1] != self.row_len():
+            raise ShapeError(L.ZP_ERR_SHAPE,
+                             f"Input length {r.shape[1] if r.dim() == 3 else -1} does not match "
+                             f"expected row length {self.row_len()}")
+        r = r.contiguous()
+        out = torch.empty((r.shape[0], self.codeword_len(), r.shape[2]), dtype=torch.int64,
+                          device=r.device)
+        _check(self._lib.zp_encode_wide(self._h, r.shape[2], _dev_ptr(r), r.shape[0],
+                                        _dev_ptr(out), _stream(stream)))
+        return out[0] if single else out
+
     def autotune(self, types, num_rows, batch=1, stream=None):
         """zp_autotune: time this code's kernel variants at a shape, keep the fastest."""
         _check(self._lib.zp_autotune(self._h, types.code, types.degree_plus_one, num_rows, batch,

@@ -72,6 +72,7 @@ SIGNATURES = {
     "zp_code_kind_of": (_int, [_c_void_p]),
     "zp_cw_elem_bytes": (_u64, [_c_void_p, _int, _u32]),
     "zp_encode_rows": (_int, [_c_void_p, _int, _u32, _c_void_p, _u64, _c_void_p, _c_void_p]),
+    "zp_encode_wide": (_int, [_c_void_p, _u32, _c_void_p, _u64, _c_void_p, _c_void_p]),
     "zp_merkle_root": (_int, [_c_void_p, _u64, _u64, _u32, _c_void_p, _c_void_p]),
     "zp_commit": (_int, [_c_void_p, _int, _u32, _c_void_p, _u64, _u32,
                          ctypes.POINTER(_c_void_p), _c_void_p, _c_void_p]),

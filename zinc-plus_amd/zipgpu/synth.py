@@ -45,6 +45,19 @@ def i128_words(seed, n):
     return splitmix64(seed, 2 * n).view(np.int64).reshape(n, 2)
 
 
+def comb_rows(seed, n, limbs, small_words=0):
+    """n CombR = Int<limbs> values as int64 [n, limbs] little-endian limbs.
+
+    small_words = 0: uniform over all 2^(64 limbs) bit patterns.  Otherwise
+    uniform signed values of small_words 64-bit words, sign-extended (the
+    range of a combined row built from i128 sums)."""
+    if small_words == 0:
+        return splitmix64(seed, n * limbs).view(np.int64).reshape(n, limbs)
+    w = splitmix64(seed, n * small_words).view(np.int64).reshape(n, small_words)
+    ext = np.where(w[:, -1:] < 0, np.int64(-1), np.int64(0))
+    return np.concatenate([w, np.repeat(ext, limbs - small_words, axis=1)], axis=1)
+
+
 def row_split(kind, n, max_iprs_row=1 << 14):
     """Row length of a commitment of n evals.
 
