@@ -264,6 +264,61 @@ int zp_combined_row(uint32_t degree_plus_one, uint32_t comb_limbs, uint64_t num_
                     const int64_t* d_alphas, const int64_t* d_coeffs, uint64_t* d_out,
                     zp_stream stream);
 
+/* ---- Fiat-Shamir transcript and proof stream ------------------------- */
+
+/* PcsProverTranscript (zip-plus/src/pcs_transcript.rs:40-221): a
+ * Blake3Transcript (transcript/src/lib.rs:10-72) plus the proof byte
+ * stream.  Not thread-safe; one owner at a time. */
+typedef struct zp_transcript zp_transcript;
+
+/* Blake3Transcript::new with an empty stream.  PcsProverTranscript::
+ * new_from_commitment(s) is this followed by zp_transcript_absorb_slice of
+ * each root. */
+int zp_transcript_create(zp_transcript** out);
+int zp_transcript_clone(const zp_transcript* t, zp_transcript** out); /* Clone */
+void zp_transcript_destroy(zp_transcript* t);
+
+/* Transcript::absorb_slice (transcript/src/traits.rs:388-392). */
+int zp_transcript_absorb_slice(zp_transcript* t, const uint8_t* bytes, uint64_t len);
+
+/* Transcript::get_challenge::<T> with T::NUM_BYTES = num_bytes: the
+ * little-endian bytes of T (lib.rs:48-55). */
+int zp_transcript_challenge(zp_transcript* t, uint8_t* h_out, uint32_t num_bytes);
+
+/* PcsProverTranscript::squeeze_challenge_idx (pcs_transcript.rs:31-36):
+ * a u32 challenge mod cap; cap = 0 is ZP_ERR_SHAPE. */
+int zp_transcript_squeeze_idx(zp_transcript* t, uint64_t cap, uint64_t* idx);
+
+/* Transcript::get_random_field_cfg::<MontyField<4>, Uint<4>, MillerRabin>
+ * (traits.rs:371-382, lib.rs:57-72): the 256-bit prime modulus,
+ * little-endian. */
+int zp_transcript_random_field_cfg(zp_transcript* t, uint8_t h_modulus[32]);
+
+/* Appends raw bytes to the proof stream without absorbing them
+ * (PcsProverTranscript::write_const_many of bytes). */
+int zp_transcript_write(zp_transcript* t, const uint8_t* bytes, uint64_t len);
+
+/* The proof bytes (PcsProverTranscript::stream). */
+uint64_t zp_transcript_stream_len(const zp_transcript* t);
+int zp_transcript_stream(const zp_transcript* t, uint8_t* h_out, uint64_t cap);
+
+/* ---- Opening proof (zip-plus/src/pcs/phase_prove.rs:85-304) ----------- */
+
+/* ZipPlus::prove::<MontyField<4>, _> of `batch` committed polys at an
+ * i128 point (num_vars x 2 little-endian i64 words), with the hint of
+ * their zp_commit, CombR = Int<comb_limbs> and num_openings column
+ * openings (ZipTypes::NUM_COLUMN_OPENINGS).  Draws alphas, writes b,
+ * draws s, writes the combined row and the openings into `t`, and returns
+ * the evaluation <q_0, b> as its canonical value (32 bytes little-endian).
+ * Errors: ZP_ERR_INVALID_PARAM for a point of the wrong length (the
+ * reference's InvalidPcsParam) or a modulus that is even or below 2^128;
+ * ZP_ERR_SHAPE for the linear-combination bit budget (validate_input's
+ * assert) or a hint of another shape.  Synchronous. */
+int zp_prove(const zp_code* code, int eval_type, uint32_t degree_plus_one, uint32_t comb_limbs,
+             const void* d_evals, uint64_t num_rows, uint32_t batch, const zp_hint* hint,
+             const int64_t* h_point, uint32_t num_vars, const uint8_t h_modulus[32],
+             uint32_t num_openings, zp_transcript* t, uint8_t h_eval[32], zp_stream stream);
+
 #ifdef __cplusplus
 }
 #endif

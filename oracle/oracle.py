@@ -289,3 +289,262 @@ def sign_extend_limbs(words, L):
     ext = np.where(w[..., -1:] < 0, np.int64(-1), np.int64(0))
     pad = np.repeat(ext, L - w.shape[-1], axis=-1)
     return np.concatenate([w, pad], axis=-1).view(np.uint64)
+
+
+# ---------------------------------------------------------------------------
+# Opening path: transcript, field, prove and verify (test infrastructure).
+# ---------------------------------------------------------------------------
+
+def blake3_xof(data, n):
+    """BLAKE3 finalize_xof().fill of n bytes over `data`."""
+    b = np.frombuffer(bytes(data), dtype=np.uint8) if len(data) else np.zeros(1, np.uint8)
+    out = np.zeros(max(1, n), dtype=np.uint8)
+    load().or_blake3_xof(_p(b), _u64(len(data)), _p(out), _u64(n))
+    return out[:n].tobytes()
+
+
+def sprp2(n):
+    """Strong probable prime to base 2 (primality/src/lib.rs:9-20)."""
+    if n < 5 or n % 2 == 0:
+        return n in (2, 3)
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    x = pow(2, d, n)
+    if x in (1, n - 1):
+        return True
+    for _ in range(s - 1):
+        x = x * x % n
+        if x == n - 1:
+            return True
+        if x == 1:
+            return False
+    return False
+
+
+class Transcript:
+    """Blake3Transcript (transcript/src/lib.rs:10-72) with the proof stream of
+    PcsProverTranscript (zip-plus/src/pcs_transcript.rs:40-221).  The hash
+    state is the whole absorbed byte string, rehashed at each challenge."""
+
+    def __init__(self, stream=b""):
+        self.absorbed = bytearray()
+        self.stream = bytearray(stream)
+        self.pos = 0
+
+    def absorb(self, b):
+        self.absorbed += bytes(b)
+
+    def absorb_slice(self, b):  # traits.rs:388-392
+        self.absorb(b"\x06" + bytes(b) + b"\x07")
+
+    def challenge(self, n):  # lib.rs:48-55
+        out = blake3_xof(self.absorbed, n)
+        self.absorb(b"\x12" + out + b"\x34")
+        return out
+
+    def challenge_i128(self):
+        return int.from_bytes(self.challenge(16), "little", signed=True)
+
+    def squeeze_idx(self, cap):  # pcs_transcript.rs:31-36
+        return int.from_bytes(self.challenge(4), "little") % cap
+
+    def random_field_cfg(self):  # traits.rs:371-382, lib.rs:57-72 (gen_random)
+        while True:
+            buf = blake3_xof(self.absorbed, 32)
+            self.absorb(buf)
+            c = int.from_bytes(buf, "little")
+            if c == 0:
+                continue
+            if c % 2 == 0:
+                c -= 1
+            if sprp2(c):
+                return c
+
+    def absorb_field(self, p, inner):  # traits.rs:394-412
+        self.absorb(b"\x03" + p.to_bytes(32, "little") + b"\x05" + b"\x01" +
+                    inner.to_bytes(32, "little") + b"\x03")
+
+    def write_field_elements(self, p, values):  # pcs_transcript.rs:99-142
+        for v in values:
+            inner = (v << 256) % p
+            self.absorb_field(p, inner)
+            self.stream += p.to_bytes(32, "little") + inner.to_bytes(32, "little")
+
+    def read(self, n):
+        if self.pos + n > len(self.stream):
+            raise ValueError("proof stream too short")
+        b = bytes(self.stream[self.pos:self.pos + n])
+        self.pos += n
+        return b
+
+    def read_field_elements(self, n):  # pcs_transcript.rs:238-283
+        out = []
+        for _ in range(n):
+            p = int.from_bytes(self.read(32), "little")
+            inner = int.from_bytes(self.read(32), "little")
+            self.absorb_field(p, inner)
+            out.append((p, inner * pow(1 << 256, -1, p) % p))
+        return out
+
+
+def _limbs4(v):
+    return np.array([(v >> (64 * i)) & ((1 << 64) - 1) for i in range(4)], dtype=np.uint64)
+
+
+def _int4(a):
+    return sum(int(a[i]) << (64 * i) for i in range(4))
+
+
+def _i128_words(values):
+    out = np.zeros((len(values), 2), dtype=np.int64)
+    for i, v in enumerate(values):
+        u = v & ((1 << 128) - 1)
+        out[i, 0] = np.uint64(u & ((1 << 64) - 1)).view(np.int64)
+        out[i, 1] = np.uint64(u >> 64).view(np.int64)
+    return out
+
+
+def eq_table(r, p):
+    """build_eq_x_r (poly/src/utils.rs:140-190): t[i] = prod_k (bit k of i ?
+    r_k : 1 - r_k) mod p."""
+    t = [1]
+    for rk in r:
+        t = [x * (1 - rk) % p for x in t] + [x * rk % p for x in t]
+    return t
+
+
+def point_to_tensor(num_rows, point, p):
+    """pcs/utils.rs:93-114: q_1 over the first coordinates (columns), q_0
+    over the last log2(num_rows) (rows)."""
+    k = len(point) - (num_rows.bit_length() - 1)
+    return eq_table(point[k:], p), eq_table(point[:k], p)
+
+
+def prove_b(p, evals, alphas, q1, num_rows, row_len, batch, D, eval_type):
+    """b_j of prove_f (phase_prove.rs:176-198), canonical ints."""
+    binpoly = eval_type == "binpoly"
+    ev = np.ascontiguousarray(evals, dtype=np.uint64 if binpoly else np.int32)
+    al = _i128_words([a for poly in alphas for a in poly]) if binpoly else np.zeros(2, np.int64)
+    q = np.stack([_limbs4(v) for v in q1])
+    out = np.zeros((num_rows, 4), dtype=np.uint64)
+    rc = load().or_prove_b(ctypes.c_int(int(binpoly)), _u64(D), _u64(num_rows), _u64(row_len),
+                           _u64(batch), _p(ev), _p(al), _p(q), _p(_limbs4(p)), _p(out))
+    if rc:
+        raise ValueError(f"oracle rc {rc}")
+    return [_int4(out[j]) for j in range(num_rows)]
+
+
+def combined_row_i32(evals, coeffs, num_rows, row_len, batch=1, L=3):
+    ev = np.ascontiguousarray(evals, dtype=np.int32)
+    co = np.ascontiguousarray(coeffs, dtype=np.int64)
+    out = np.zeros((row_len, L), dtype=np.uint64)
+    rc = load().or_combined_row_i32(_u64(L), _u64(num_rows), _u64(row_len), _u64(batch), _p(ev),
+                                    _p(co), _p(out))
+    if rc:
+        raise ValueError(f"oracle rc {rc}")
+    return out
+
+
+def merkle_prove(layers, n, leaf):
+    levels = n.bit_length() - 1
+    sib = np.zeros((max(1, levels), 32), dtype=np.uint8)
+    count = ctypes.c_uint64()
+    lay = np.ascontiguousarray(layers, dtype=np.uint8)
+    rc = load().or_merkle_prove(_p(lay), _u64(n), _u64(leaf), _p(sib), ctypes.byref(count))
+    if rc:
+        raise ValueError(f"oracle rc {rc}")
+    return [sib[i].tobytes() for i in range(count.value)]
+
+
+def prove(ts, p, evals, num_rows, row_len, N, elem, cw, layers, point, batch=1, D=32, L=5,
+          eval_type="binpoly", num_openings=147):
+    """ZipPlus::prove_f (zip-plus/src/pcs/phase_prove.rs:126-253) on a
+    transcript that already holds the commitment and the field config.
+    Returns the evaluation (canonical int); the proof bytes are ts.stream."""
+    binpoly = eval_type == "binpoly"
+    nv = (num_rows * row_len).bit_length() - 1
+    if len(point) != nv:
+        raise ValueError("Invalid point")
+    pt = [v % p for v in point]
+    q0, q1 = point_to_tensor(num_rows, pt, p)
+    alphas = [[ts.challenge_i128() for _ in range(D)] if binpoly and D > 1 else [1]
+              for _ in range(batch)]
+    b = prove_b(p, evals, alphas, q1, num_rows, row_len, batch, D, eval_type)
+    ts.write_field_elements(p, b)
+    ev = sum(x * y for x, y in zip(q0, b)) % p
+    coeffs = [1] if num_rows == 1 else [ts.challenge_i128() for _ in range(num_rows)]
+    cw_words = _i128_words(coeffs)
+    if binpoly:
+        comb = combined_row(evals, _i128_words([a for poly in alphas for a in poly]), cw_words,
+                            num_rows, row_len, batch, D, L)
+    else:
+        comb = combined_row_i32(evals, cw_words, num_rows, row_len, batch, L)
+    ts.stream += comb.tobytes()
+    cwb = np.ascontiguousarray(cw, dtype=np.uint8).reshape(batch * num_rows, N, elem)
+    levels = N.bit_length() - 1
+    for _ in range(num_openings):
+        idx = ts.squeeze_idx(N)
+        ts.stream += cwb[:, idx, :].tobytes()
+        ts.stream += np.array([idx, N, levels], dtype=np.uint64).tobytes()
+        ts.stream += b"".join(merkle_prove(layers, N, idx))
+    return ev
+
+
+def _signed(words):
+    v = 0
+    for i, w in enumerate(words):
+        v |= int(w) << (64 * i)
+    n = 64 * len(words)
+    return v - (1 << n) if v >> (n - 1) else v
+
+
+def verify(ts, p, root, point, ev, encode_wide, num_rows, row_len, N, elem, batch=1, D=32, L=5,
+           eval_type="binpoly", coeff_bytes=8, num_openings=147):
+    """ZipPlus::verify (zip-plus/src/pcs/phase_verify.rs:100-305) on a
+    verifier transcript (root absorbed, field config drawn, stream at the
+    proof).  encode_wide(uint64 [row_len, L]) -> uint64 [N, L].  Raises
+    ValueError naming the failed check."""
+    binpoly = eval_type == "binpoly"
+    alphas = [[ts.challenge_i128() for _ in range(D)] if binpoly and D > 1 else [1]
+              for _ in range(batch)]
+    pt = [v % p for v in point]
+    q0, q1 = point_to_tensor(num_rows, pt, p)
+    b = ts.read_field_elements(num_rows)
+    if any(m != p for m, _ in b):
+        raise ValueError("modulus mismatch")
+    b = [v for _, v in b]
+    if sum(x * y for x, y in zip(q0, b)) % p != ev % p:
+        raise ValueError("Evaluation consistency failure")
+    coeffs = [1] if num_rows == 1 else [ts.challenge_i128() for _ in range(num_rows)]
+    comb = np.frombuffer(ts.read(row_len * L * 8), dtype=np.uint64).reshape(row_len, L)
+    enc = encode_wide(comb)
+    lhs = sum(_signed(comb[c]) * q1[c] for c in range(row_len)) % p
+    rhs = sum(s * x for s, x in zip(coeffs, b)) % p
+    if lhs != rhs:
+        raise ValueError("Coherence failure")
+    mask = (1 << (64 * L)) - 1
+    levels = N.bit_length() - 1
+    for _ in range(num_openings):
+        idx = ts.squeeze_idx(N)
+        col = ts.read(batch * num_rows * elem)
+        leaf, count, nsib = np.frombuffer(ts.read(24), dtype=np.uint64).tolist()
+        sib = [ts.read(32) for _ in range(nsib)]
+        if leaf != idx or count != N or nsib != levels:
+            raise ValueError("Merkle proof header mismatch")
+        acc = 0
+        for pb in range(batch):
+            for j in range(num_rows):
+                e = col[(pb * num_rows + j) * elem:(pb * num_rows + j + 1) * elem]
+                ncoef = elem // coeff_bytes
+                coefs = [int.from_bytes(e[k * coeff_bytes:(k + 1) * coeff_bytes], "little",
+                                        signed=True) for k in range(ncoef)]
+                acc += coeffs[j] * sum(c * a for c, a in zip(coefs, alphas[pb]))
+        if acc & mask != _signed(enc[idx]) & mask:
+            raise ValueError("Proximity failure")
+        if not merkle_verify(root, col, idx, N, sib):
+            raise ValueError("Column opening verification failed")
+    if ts.pos != len(ts.stream):
+        raise ValueError("trailing proof bytes")
+    return True

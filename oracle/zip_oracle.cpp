@@ -1077,3 +1077,252 @@ extern "C" int or_raa_encode_wide(uint64_t row_len, uint64_t rep, int sign_flip,
   });
   return OR_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Opening path.
+//
+// BLAKE3 extended output (finalize_xof) of a whole input: the root node's
+// compression repeated with output block counters 0, 1, ... (BLAKE3 spec
+// section 2.6).  The Fiat-Shamir transcript (transcript/src/lib.rs:10-72)
+// reads its challenges this way from the hash of everything absorbed.
+// ---------------------------------------------------------------------------
+static void b3_root_block(const uint8_t* data, uint64_t len, uint64_t counter, uint32_t o[16]) {
+  const uint64_t nchunks = len == 0 ? 1 : (len + 1023) / 1024;
+  if (nchunks == 1) {
+    uint32_t cv[8];
+    std::memcpy(cv, B3_IV, 32);
+    const uint64_t nblocks = len == 0 ? 1 : (len + 63) / 64;
+    for (uint64_t b = 0; b < nblocks; b++) {
+      const uint64_t off = b * 64, blen = std::min<uint64_t>(64, len - off);
+      uint32_t m[16];
+      load_block(data + off, blen, m);
+      uint32_t flags = b == 0 ? CHUNK_START : 0;
+      if (b + 1 < nblocks) {
+        uint32_t t[16];
+        b3_compress(cv, m, 0, (uint32_t)blen, flags, t);
+        std::memcpy(cv, t, 32);
+      } else {
+        b3_compress(cv, m, counter, (uint32_t)blen, flags | CHUNK_END | ROOT, o);
+      }
+    }
+    return;
+  }
+  uint64_t left = 1;
+  while (left * 2 < nchunks) left *= 2;
+  uint32_t m[16];
+  b3_subtree(data, len, 0, left, false, m);
+  b3_subtree(data, len, left, nchunks - left, false, m + 8);
+  b3_compress(B3_IV, m, counter, 64, PARENT | ROOT, o);
+}
+
+extern "C" int or_blake3_xof(const uint8_t* data, uint64_t len, uint8_t* out, uint64_t n) {
+  for (uint64_t blk = 0; blk * 64 < n; blk++) {
+    uint32_t o[16];
+    b3_root_block(data, len, blk, o);
+    const uint64_t take = std::min<uint64_t>(64, n - blk * 64);
+    std::memcpy(out + blk * 64, o, take);  // little-endian words
+  }
+  return OR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Prime field F_p, p < 2^256, by definition: residues kept canonical,
+// products and sums reduced by binary long division (no Montgomery
+// arithmetic).  The proof stream carries an element a as a * 2^256 mod p,
+// which is also computed by division here.
+// ---------------------------------------------------------------------------
+// r = x mod p for an n-limb unsigned x.
+static void big_mod(const uint64_t* x, int n, const uint64_t p[4], uint64_t r[4]) {
+  uint64_t acc[5] = {0, 0, 0, 0, 0};  // < 2p < 2^257
+  for (int bit = 64 * n - 1; bit >= 0; bit--) {
+    for (int i = 4; i > 0; i--) acc[i] = (acc[i] << 1) | (acc[i - 1] >> 63);
+    acc[0] = (acc[0] << 1) | ((x[bit / 64] >> (bit % 64)) & 1);
+    bool ge = acc[4] != 0;
+    if (!ge) {
+      ge = true;
+      for (int i = 3; i >= 0; i--)
+        if (acc[i] != p[i]) {
+          ge = acc[i] > p[i];
+          break;
+        }
+    }
+    if (ge) {
+      uint64_t br = 0;
+      for (int i = 0; i < 5; i++) {
+        const uint64_t pi = i < 4 ? p[i] : 0;
+        const u128 d = (u128)acc[i] - pi - br;
+        acc[i] = (uint64_t)d;
+        br = (uint64_t)(d >> 64) & 1;
+      }
+    }
+  }
+  std::memcpy(r, acc, 32);
+}
+
+// out[na+nb] = a * b (unsigned).
+static void big_mul_u(const uint64_t* a, int na, const uint64_t* b, int nb, uint64_t* out) {
+  for (int i = 0; i < na + nb; i++) out[i] = 0;
+  for (int i = 0; i < na; i++) {
+    u128 c = 0;
+    for (int j = 0; j < nb; j++) {
+      const u128 t = (u128)a[i] * b[j] + out[i + j] + c;
+      out[i + j] = (uint64_t)t;
+      c = t >> 64;
+    }
+    out[i + nb] = (uint64_t)c;
+  }
+}
+
+static void fmod_add(const uint64_t a[4], const uint64_t b[4], const uint64_t p[4], uint64_t r[4]) {
+  uint64_t s[5];
+  u128 c = 0;
+  for (int i = 0; i < 4; i++) {
+    c += (u128)a[i] + b[i];
+    s[i] = (uint64_t)c;
+    c >>= 64;
+  }
+  s[4] = (uint64_t)c;
+  big_mod(s, 5, p, r);
+}
+
+static void fmod_sub(const uint64_t a[4], const uint64_t b[4], const uint64_t p[4], uint64_t r[4]) {
+  uint64_t nb[4], z[4] = {0, 0, 0, 0};
+  // p - b (b < p), then a + (p - b)
+  uint64_t br = 0;
+  for (int i = 0; i < 4; i++) {
+    const u128 d = (u128)p[i] - b[i] - br;
+    nb[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+  if (std::memcmp(b, z, 32) == 0) std::memcpy(nb, z, 32);
+  fmod_add(a, nb, p, r);
+}
+
+extern "C" int or_field_mul(const uint64_t* a, const uint64_t* b, const uint64_t* p, uint64_t* r) {
+  uint64_t t[8];
+  big_mul_u(a, 4, b, 4, t);
+  big_mod(t, 8, p, r);
+  return OR_OK;
+}
+
+// a * 2^256 mod p: the element's bytes in the proof stream.
+extern "C" int or_field_inner(const uint64_t* a, const uint64_t* p, uint64_t* r) {
+  uint64_t t[8] = {0, 0, 0, 0, a[0], a[1], a[2], a[3]};
+  big_mod(t, 8, p, r);
+  return OR_OK;
+}
+
+// A two's complement integer of n limbs mod p.
+static void signed_mod(const uint64_t* x, int n, const uint64_t p[4], uint64_t r[4]) {
+  const bool neg = (int64_t)x[n - 1] < 0;
+  std::vector<uint64_t> m(x, x + n);
+  if (neg) {
+    u128 c = 1;
+    for (int i = 0; i < n; i++) {
+      c += (u128)(~m[i]);
+      m[i] = (uint64_t)c;
+      c >>= 64;
+    }
+  }
+  uint64_t v[4];
+  big_mod(m.data(), n, p, v);
+  if (neg) {
+    const uint64_t z[4] = {0, 0, 0, 0};
+    fmod_sub(z, v, p, r);
+  } else {
+    std::memcpy(r, v, 32);
+  }
+}
+
+extern "C" int or_field_from_signed(const uint64_t* x, uint64_t n, const uint64_t* p, uint64_t* r) {
+  if (n == 0 || n > 16) return OR_ERR_INVALID_PARAM;
+  signed_mod(x, (int)n, p, r);
+  return OR_OK;
+}
+
+// b of prove_f (zip-plus/src/pcs/phase_prove.rs:176-198): for every row j,
+//   b_j = sum_poly sum_col F(w'_poly[j*row_len + col]) * q1[col]  (mod p),
+// with w' the alpha projection (binary evals, :153-174) or the eval itself
+// (i32 evals, ScalarProduct with alphas = [1]).  The sum of signed products
+// is formed exactly (positive and negative parts) and reduced once per row.
+// q1: [row_len][4] canonical; out: [num_rows][4] canonical.
+extern "C" int or_prove_b(int binpoly, uint64_t D, uint64_t num_rows, uint64_t row_len,
+                          uint64_t batch, const void* evals, const int64_t* alphas,
+                          const uint64_t* q1, const uint64_t* p, uint64_t* out) {
+  if (binpoly && (D == 0 || D > 64)) return OR_ERR_INVALID_PARAM;
+  parallel_for(num_rows, [&](uint64_t j) {
+    uint64_t b[4] = {0, 0, 0, 0};
+    for (uint64_t pb = 0; pb < batch; pb++) {
+      uint64_t pos[9] = {0}, neg[9] = {0};
+      for (uint64_t col = 0; col < row_len; col++) {
+        const uint64_t idx = (pb * num_rows + j) * row_len + col;
+        i128 hi = 0;  // w' = hi * 2^128 + lo, exact in 192 bits
+        u128 lo = 0;
+        if (binpoly) {
+          uint64_t bits = ((const uint64_t*)evals)[idx];
+          if (D < 64) bits &= (1ull << D) - 1;
+          while (bits) {
+            const int c = __builtin_ctzll(bits);
+            const i128 a = load_i128(alphas + 2 * (pb * D + c));
+            const u128 s = lo + (u128)a;
+            hi += (s < lo ? 1 : 0) + (a < 0 ? -1 : 0);
+            lo = s;
+            bits &= bits - 1;
+          }
+        } else {
+          const i128 a = ((const int32_t*)evals)[idx];
+          lo = (u128)a;
+          hi = a < 0 ? -1 : 0;
+        }
+        uint64_t w[3] = {(uint64_t)lo, (uint64_t)(lo >> 64), (uint64_t)hi};
+        const bool negative = (int64_t)w[2] < 0;
+        if (negative) {
+          u128 c = 1;
+          for (int i = 0; i < 3; i++) {
+            c += (u128)(~w[i]);
+            w[i] = (uint64_t)c;
+            c >>= 64;
+          }
+        }
+        uint64_t prod[7];
+        big_mul_u(w, 3, q1 + 4 * col, 4, prod);
+        uint64_t* acc = negative ? neg : pos;
+        u128 c = 0;
+        for (int i = 0; i < 9; i++) {
+          c += (u128)acc[i] + (i < 7 ? prod[i] : 0);
+          acc[i] = (uint64_t)c;
+          c >>= 64;
+        }
+      }
+      uint64_t rp[4], rn[4], d[4];
+      big_mod(pos, 9, p, rp);
+      big_mod(neg, 9, p, rn);
+      fmod_sub(rp, rn, p, d);
+      fmod_add(b, d, p, b);
+    }
+    std::memcpy(out + 4 * j, b, 32);
+  });
+  return OR_OK;
+}
+
+// Combined row of i32 evals (phase_prove.rs:205-244 with ScalarProduct
+// alpha projection: w' = eval): comb[col] = sum_poly sum_j s_j * x_j[col]
+// in Int<L>.
+extern "C" int or_combined_row_i32(uint64_t L, uint64_t num_rows, uint64_t row_len,
+                                   uint64_t batch, const int32_t* evals, const int64_t* coeffs,
+                                   uint64_t* out) {
+  if (L < 2 || L > 8) return OR_ERR_INVALID_PARAM;
+  parallel_for(row_len, [&](uint64_t col) {
+    BigL acc;
+    big_from_i128(acc, 0, (int)L);
+    for (uint64_t b = 0; b < batch; b++)
+      for (uint64_t j = 0; j < num_rows; j++) {
+        BigL x, t;
+        big_from_i128(x, evals[(b * num_rows + j) * row_len + col], (int)L);
+        big_mul_i128(t, x, load_i128(coeffs + 2 * j), (int)L);
+        big_add(acc, t, (int)L);
+      }
+    for (uint64_t i = 0; i < L; i++) out[col * L + i] = acc.w[i];
+  });
+  return OR_OK;
+}

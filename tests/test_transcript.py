@@ -1,0 +1,72 @@
+"""The library's Fiat-Shamir transcript (host code behind the C ABI) against
+the oracle's: the same absorbs give the same challenges, indices, prime and
+stream bytes (transcript/src/lib.rs:10-72, transcript/src/traits.rs:358-412,
+zip-plus/src/pcs_transcript.rs:27-221).  CPU only: no device calls."""
+import random
+
+import pytest
+
+
+@pytest.fixture(scope="module")
+def zgh():
+    import zipgpu
+    return zipgpu
+
+
+def test_challenges_and_indices_match(zgh, oracle):
+    rng = random.Random(1)
+    t, o = zgh.PcsProverTranscript(), oracle.Transcript()
+    for step in range(400):
+        op = rng.randrange(4)
+        if op == 0:
+            data = bytes(rng.randrange(256) for _ in range(rng.choice([0, 1, 31, 32, 64, 65, 700])))
+            t.absorb_slice(data)
+            o.absorb_slice(data)
+        elif op == 1:
+            n = rng.choice([1, 4, 16, 32, 33, 64, 65, 200])
+            assert t.get_challenge(n) == o.challenge(n), step
+        elif op == 2:
+            cap = rng.choice([1, 7, 1 << 16, 65536 * 3 + 1])
+            assert t.squeeze_challenge_idx(cap) == o.squeeze_idx(cap), step
+        else:
+            assert t.get_challenge_i128() == o.challenge_i128(), step
+
+
+@pytest.mark.parametrize("sizes", [[1024], [1023, 1, 1], [5000], [1024 * 7 + 3, 64, 2049],
+                                   [3] * 800, [64] * 40])
+def test_incremental_hash_across_chunks(zgh, oracle, sizes):
+    # Absorbs that fill and cross 1 KiB chunks exercise the chaining-value
+    # stack; the challenge must equal the XOF of the whole absorbed string.
+    rng = random.Random(len(sizes))
+    t, o = zgh.PcsProverTranscript(), oracle.Transcript()
+    for n in sizes:
+        data = bytes(rng.randrange(256) for _ in range(n))
+        t.absorb_slice(data)
+        o.absorb_slice(data)
+    assert t.get_challenge(131) == o.challenge(131)
+
+
+def test_random_field_cfg_matches(zgh, oracle):
+    for seed in range(4):
+        root = bytes((seed * 37 + i) % 256 for i in range(32))
+        t, o = zgh.PcsProverTranscript(), oracle.Transcript()
+        t.absorb_slice(root)
+        o.absorb_slice(root)
+        assert t.get_random_field_cfg() == o.random_field_cfg()
+        assert t.get_challenge(32) == o.challenge(32)
+
+
+def test_clone_and_stream(zgh, oracle):
+    t = zgh.PcsProverTranscript()
+    t.absorb_slice(b"root")
+    t.write(b"abc")
+    u = t.clone()
+    assert u.get_challenge(16) == t.get_challenge(16)
+    u.write(b"de")
+    assert t.stream() == b"abc" and u.stream() == b"abcde"
+
+
+def test_squeeze_zero_cap_is_an_error(zgh):
+    t = zgh.PcsProverTranscript()
+    with pytest.raises(zgh.ShapeError):
+        t.squeeze_challenge_idx(0)

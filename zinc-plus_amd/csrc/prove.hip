@@ -1,0 +1,419 @@
+// prove.hip -- ZipPlus::prove: the opening proof of committed polys.
+//
+// Reference: zip-plus/src/pcs/phase_prove.rs:85-253 (prove / prove_f),
+// :282-304 (open_merkle_trees_for_column), pcs/utils.rs:93-114
+// (point_to_tensor with poly/src/utils.rs:140-190 build_eq_x_r),
+// pcs/utils.rs:16-78 (validate_input).
+//
+// Device work (all exact):
+//   q_1, q_0 = eq(x, r) tables of the two halves of the point, one thread
+//     per entry (the product of r_k or 1 - r_k over the index bits);
+//   b_j = sum_col F(w'_j[col]) q_1[col] with w'_j[col] = sum_c alpha_c
+//     bit_c(x_j[col]) (an exact integer: |w'| < 2^134), computed as
+//     b_j = sum_c alpha_c P_c[j],  P_c[j] = sum_{col: bit c set} q_1[col]:
+//     one workgroup per row sums the Montgomery-form q_1 entries of every
+//     plane exactly (288-bit accumulators) and reduces once;
+//   the combined row (combined.hip) and the column / Merkle-path gathers.
+// The host runs the transcript: alphas, the b elements, eval = <q_0, b>,
+// the coefficients s, the column indices, and the proof bytes.
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "field.h"
+
+namespace zpi {
+
+void ts_challenge(zp_transcript* t, uint8_t* out, size_t n);
+uint64_t ts_squeeze_idx(zp_transcript* t, uint64_t cap);
+void ts_write_field_elements(zp_transcript* t, const uint64_t mod[4], const Fp* v, size_t n);
+void ts_put(zp_transcript* t, const void* p, size_t n);
+
+namespace {
+
+// out[i] = prod_k (bit k of i ? r_k : 1 - r_k); rr = [r_0, 1 - r_0, r_1, ...].
+__global__ void __launch_bounds__(256)
+eq_table_kernel(const Fp* __restrict__ rr, uint32_t nv, Fp* __restrict__ out, FpParams P) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (1ull << nv)) return;
+  Fp acc = fp_one(P);
+  for (uint32_t k = 0; k < nv; k++) acc = fp_mul(P, acc, rr[2 * k + (((i >> k) & 1) ? 0 : 1)]);
+  out[i] = acc;
+}
+
+constexpr int kSumWaves = 4;
+
+// One workgroup per (poly, row): P[row][c] = sum of q1[col] over the columns
+// whose eval has bit c set, reduced mod p.  Lane = (plane c, sub-column);
+// lanes of one wave read 64/Dp adjacent columns.
+template <typename Word>
+__global__ void __launch_bounds__(64 * kSumWaves)
+plane_sums_kernel(const Word* __restrict__ evals, const Fp* __restrict__ q1, Fp* __restrict__ out,
+                  uint64_t row_len, uint32_t D, uint32_t dp_log2, uint32_t red_bits, FpParams P) {
+  __shared__ uint32_t part[64 * kSumWaves][9];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t Dp = 1u << dp_log2;
+  const uint32_t c = lane & (Dp - 1), sub = lane >> dp_log2;
+  const uint32_t per_step = (64u >> dp_log2) * kSumWaves;
+  const Word* x = evals + (uint64_t)blockIdx.x * row_len;
+  uint32_t acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint64_t col = (uint64_t)wave * (64u >> dp_log2) + sub; col < row_len; col += per_step) {
+    const uint32_t mask = 0u - (uint32_t)((x[col] >> c) & 1u);
+    const uint32_t* q = (const uint32_t*)(q1 + col);
+    uint64_t carry = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      const uint64_t s = (uint64_t)acc[k] + (q[k] & mask) + carry;
+      acc[k] = (uint32_t)s;
+      carry = s >> 32;
+    }
+    acc[8] += (uint32_t)carry;
+  }
+#pragma unroll
+  for (int k = 0; k < 9; k++) part[threadIdx.x][k] = acc[k];
+  __syncthreads();
+  if (threadIdx.x >= D) return;
+  // Sum the partials of plane c: lanes c + m*Dp of every wave.
+  uint32_t s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t t = threadIdx.x; t < 64 * kSumWaves; t += Dp) {
+    uint64_t carry = 0;
+    for (int k = 0; k < 9; k++) {
+      const uint64_t v = (uint64_t)s[k] + part[t][k] + carry;
+      s[k] = (uint32_t)v;
+      carry = v >> 32;
+    }
+  }
+  // s < 2^red_bits * p: subtract p * 2^k where it fits, k = red_bits-1 .. 0.
+  for (int k = (int)red_bits - 1; k >= 0; k--) {
+    uint32_t pk[9];
+    const uint32_t* pw = (const uint32_t*)P.p;
+    for (int i = 0; i < 9; i++) {
+      const uint32_t lo = i < 8 ? pw[i] : 0;
+      const uint32_t lo_prev = i > 0 ? pw[i - 1] : 0;
+      pk[i] = k == 0 ? lo : (lo << k) | (i > 0 ? lo_prev >> (32 - k) : 0);
+    }
+    bool ge = true;
+    for (int i = 8; i >= 0; i--) {
+      if (s[i] != pk[i]) {
+        ge = s[i] > pk[i];
+        break;
+      }
+    }
+    if (ge) {
+      uint64_t br = 0;
+      for (int i = 0; i < 9; i++) {
+        const uint64_t d = (uint64_t)s[i] - pk[i] - br;
+        s[i] = (uint32_t)d;
+        br = (d >> 32) & 1;
+      }
+    }
+  }
+  Fp r;
+  for (int i = 0; i < 4; i++) r.w[i] = (uint64_t)s[2 * i] | ((uint64_t)s[2 * i + 1] << 32);
+  out[(uint64_t)blockIdx.x * D + threadIdx.x] = r;
+}
+
+// b[j] = sum_b sum_c alpha_{b,c} * P[b][j][c] (all Montgomery form).
+__global__ void __launch_bounds__(256)
+b_kernel(const Fp* __restrict__ planes, const Fp* __restrict__ alphas, Fp* __restrict__ b,
+         uint64_t num_rows, uint32_t D, uint32_t batch, FpParams P) {
+  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= num_rows) return;
+  Fp acc = {{0, 0, 0, 0}};
+  for (uint32_t p = 0; p < batch; p++)
+    for (uint32_t c = 0; c < D; c++)
+      acc = fp_add(P, acc, fp_mul(P, alphas[p * D + c], planes[((uint64_t)p * num_rows + j) * D + c]));
+  b[j] = acc;
+}
+
+// i32 evals as u64 lanes (bit pattern zero-extended) for the combined row.
+__global__ void widen_i32_kernel(const uint32_t* __restrict__ in, uint64_t* __restrict__ out,
+                                 uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = in[i];
+}
+
+// Column openings: out[(k*rows + r)*words + w] = cw[(r*N + col_k)*words + w].
+__global__ void gather_columns_kernel(const uint32_t* __restrict__ cw,
+                                      const uint64_t* __restrict__ cols, uint32_t* __restrict__ out,
+                                      uint64_t rows, uint64_t N, uint32_t words, uint32_t n_cols) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)n_cols * rows * words;
+  if (idx >= total) return;
+  const uint32_t w = (uint32_t)(idx % words);
+  const uint64_t r = (idx / words) % rows;
+  const uint64_t k = idx / words / rows;
+  out[idx] = cw[(r * N + cols[k]) * words + w];
+}
+
+// Merkle paths: out[(k*levels + l)*8 + w] = word w of the sibling of column
+// k's ancestor at level l (layers: leaves first, 32 bytes per node).
+__global__ void gather_paths_kernel(const uint32_t* __restrict__ layers,
+                                    const uint64_t* __restrict__ cols, uint32_t* __restrict__ out,
+                                    uint64_t N, uint32_t levels, uint32_t n_cols) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t total = (uint64_t)n_cols * levels * 8;
+  if (idx >= total) return;
+  const uint32_t w = (uint32_t)(idx % 8);
+  const uint32_t l = (uint32_t)((idx / 8) % levels);
+  const uint64_t k = idx / 8 / levels;
+  const uint64_t off = 2 * N - 2 * (N >> l);  // first node of level l
+  const uint64_t node = (cols[k] >> l) ^ 1;
+  out[idx] = layers[(off + node) * 8 + w];
+}
+
+uint32_t ceil_log2(uint64_t x) {
+  uint32_t r = 0;
+  while ((1ull << r) < x) r++;
+  return r;
+}
+
+// Device buffers of one proof, carved from one allocation.
+struct Bufs {
+  std::vector<std::pair<void**, uint64_t>> parts;
+  void* base = nullptr;
+  void add(void** p, uint64_t bytes) { parts.push_back({p, (bytes + 255) & ~255ull}); }
+  uint64_t total() const {
+    uint64_t t = 0;
+    for (auto& p : parts) t += p.second;
+    return t;
+  }
+  int alloc(hipStream_t s) {
+    ZP_TRY(dev_alloc(&base, std::max<uint64_t>(total(), 256), s));
+    uint8_t* q = (uint8_t*)base;
+    for (auto& p : parts) {
+      *p.first = q;
+      q += p.second;
+    }
+    return ZP_OK;
+  }
+};
+
+}  // namespace
+
+int launch_gather_openings(const zp_hint* h, const uint64_t* d_cols, uint32_t n_cols,
+                           void* d_columns, void* d_paths, hipStream_t s) {
+  if (!n_cols) return ZP_OK;
+  const uint64_t rows = h->num_rows * h->batch, words = h->elem / 4;
+  const uint32_t levels = h->height - 1;
+  const uint64_t tc = n_cols * rows * words;
+  hipLaunchKernelGGL(gather_columns_kernel, dim3((uint32_t)((tc + 255) / 256)), dim3(256), 0, s,
+                     (const uint32_t*)h->d_cw, d_cols, (uint32_t*)d_columns, rows, h->N,
+                     (uint32_t)words, n_cols);
+  ZP_HIP(hipGetLastError());
+  if (levels) {
+    const uint64_t tp = (uint64_t)n_cols * levels * 8;
+    hipLaunchKernelGGL(gather_paths_kernel, dim3((uint32_t)((tp + 255) / 256)), dim3(256), 0, s,
+                       (const uint32_t*)h->d_layers, d_cols, (uint32_t*)d_paths, h->N, levels,
+                       n_cols);
+    ZP_HIP(hipGetLastError());
+  }
+  return ZP_OK;
+}
+
+}  // namespace zpi
+
+using namespace zpi;
+
+extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t L,
+                        const void* d_evals, uint64_t num_rows, uint32_t batch,
+                        const zp_hint* hint, const int64_t* h_point, uint32_t num_vars,
+                        const uint8_t h_modulus[32], uint32_t num_openings, zp_transcript* t,
+                        uint8_t h_eval[32], zp_stream stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!code || !hint || !t || !h_modulus || !h_eval || (num_vars && !h_point) || !d_evals)
+    return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  ZP_TRY(check_types(code, eval_type, D));
+  if (L < 3 || L > 8) return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
+  const uint64_t row_len = code->row_len, N = code->N;
+  if (batch == 0) return fail(ZP_ERR_SHAPE, "Batch must contain at least one polynomial");
+  if (!pow2(num_rows)) return fail(ZP_ERR_SHAPE, "num_rows must be a power of two");
+  if (hint->N != N || hint->num_rows != num_rows || hint->batch != batch ||
+      hint->elem != cw_elem_bytes_of(code, eval_type, D))
+    return fail(ZP_ERR_SHAPE, "hint does not match the code, rows, batch or eval type");
+  // validate_input (pcs/utils.rs:16-78): the combined row's bit budget, and
+  // the point has one coordinate per variable of the polys.
+  {
+    const bool binpoly = eval_type == ZP_EVAL_BINPOLY;
+    const uint64_t d = (uint64_t)__builtin_ctzll(row_len);
+    const uint64_t cw_bits = ceil_log2(std::max<uint64_t>(code->rep * d, 1));
+    const uint64_t chal_bits = (binpoly && D > 1) ? 256 : 128;
+    const uint64_t coeff_bits = binpoly ? 1 : 32;
+    const uint64_t lc = 2 * cw_bits + ceil_log2(std::max<uint64_t>(d, 1)) + chal_bits +
+                        (coeff_bits - 1) + ceil_log2(batch);
+    if (lc > 64ull * L)
+      return fail(ZP_ERR_SHAPE, "The number of bits used for linear combinations is too large: " +
+                                    std::to_string(lc) + " bits, max allowed is " +
+                                    std::to_string(64 * L) + " bits");
+    const uint64_t nv = __builtin_ctzll(num_rows) + d;
+    if (num_vars != nv)
+      return fail(ZP_ERR_INVALID_PARAM, "Invalid point (expect point to have " +
+                                            std::to_string(nv) + " variates but got " +
+                                            std::to_string(num_vars) + ")");
+  }
+  uint64_t mod[4];
+  std::memcpy(mod, h_modulus, 32);
+  if (!(mod[0] & 1) || (mod[2] | mod[3]) == 0)
+    return fail(ZP_ERR_INVALID_PARAM, "field modulus must be odd and above 2^128");
+  const FpParams P = fp_params(mod);
+  const bool binpoly = eval_type == ZP_EVAL_BINPOLY;
+  const uint32_t Dk = binpoly ? D : 32;  // planes of one eval
+  const uint32_t row_vars = __builtin_ctzll(num_rows), col_vars = num_vars - row_vars;
+
+  // Point in the field: hi = point[..col_vars] -> q_1, lo -> q_0.
+  std::vector<Fp> rr(2 * (size_t)num_vars);
+  for (uint32_t k = 0; k < num_vars; k++) {
+    const Fp r = fp_from_signed_limbs(P, (const uint64_t*)(h_point + 2 * k), 2);
+    rr[2 * k] = r;
+    rr[2 * k + 1] = fp_sub(P, fp_one(P), r);
+  }
+  // Per-poly alphas (get_challenges::<i128>(degree_bound + 1)), or [1].
+  std::vector<int64_t> al((size_t)batch * Dk * 2);
+  std::vector<Fp> alm((size_t)batch * Dk);
+  for (uint32_t b = 0; b < batch; b++) {
+    for (uint32_t c = 0; c < Dk; c++) {
+      int64_t* a = &al[((size_t)b * Dk + c) * 2];
+      if (binpoly && D > 1) {
+        ts_challenge(t, (uint8_t*)a, 16);
+      } else if (binpoly) {
+        a[0] = 1, a[1] = 0;
+      } else {  // i32 eval as bit planes: bit c weighs 2^c, bit 31 weighs -2^31
+        a[0] = c < 31 ? (int64_t)(1ull << c) : INT64_MIN >> 32;
+        a[1] = c < 31 ? 0 : -1;
+      }
+      alm[(size_t)b * Dk + c] = fp_from_signed_limbs(P, (const uint64_t*)a, 2);
+    }
+  }
+
+  const uint64_t rows = num_rows * batch, elem = hint->elem;
+  const uint32_t levels = hint->height - 1;
+  Fp *d_rr, *d_q1, *d_q0, *d_planes, *d_b, *d_alm;
+  int64_t *d_al, *d_co;
+  uint64_t *d_cols, *d_comb, *d_wide = nullptr;
+  void *d_scr, *d_columns, *d_paths;
+  Bufs bufs;
+  bufs.add((void**)&d_rr, rr.size() * sizeof(Fp));
+  bufs.add((void**)&d_q1, row_len * sizeof(Fp));
+  bufs.add((void**)&d_q0, num_rows * sizeof(Fp));
+  bufs.add((void**)&d_planes, rows * Dk * sizeof(Fp));
+  bufs.add((void**)&d_b, num_rows * sizeof(Fp));
+  bufs.add((void**)&d_alm, alm.size() * sizeof(Fp));
+  bufs.add((void**)&d_al, al.size() * 8);
+  bufs.add((void**)&d_co, num_rows * 16);
+  bufs.add((void**)&d_cols, (uint64_t)num_openings * 8);
+  bufs.add((void**)&d_comb, row_len * L * 8);
+  bufs.add(&d_scr, combined_row_scratch_bytes(Dk, row_len, batch, num_rows));
+  bufs.add(&d_columns, (uint64_t)num_openings * rows * elem);
+  bufs.add(&d_paths, (uint64_t)num_openings * levels * 32);
+  if (!binpoly) bufs.add((void**)&d_wide, rows * row_len * 8);
+  ZP_TRY(bufs.alloc(s));
+  int rc = ZP_OK;
+  auto done = [&](int r) {
+    dev_free(bufs.base, s);
+    return r;
+  };
+  const uint64_t up1 = rr.size() * sizeof(Fp) + alm.size() * sizeof(Fp);
+  const uint64_t down1 = 2 * num_rows * sizeof(Fp);
+  const uint64_t up2 = al.size() * 8 + num_rows * 16 + (uint64_t)num_openings * 8;
+  const uint64_t down2 =
+      row_len * L * 8 + (uint64_t)num_openings * rows * elem + (uint64_t)num_openings * levels * 32;
+  uint8_t* pin = pinned_scratch(std::max(up1 + down1, up2 + down2));
+  if (!pin) return done(fail(ZP_ERR_OOM, "pinned staging allocation failed"));
+
+  // Round 1: q tables, plane sums, b.
+  std::memcpy(pin, rr.data(), rr.size() * sizeof(Fp));
+  std::memcpy(pin + rr.size() * sizeof(Fp), alm.data(), alm.size() * sizeof(Fp));
+  if (hipMemcpyAsync(d_rr, pin, rr.size() * sizeof(Fp), hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(d_alm, pin + rr.size() * sizeof(Fp), alm.size() * sizeof(Fp),
+                     hipMemcpyHostToDevice, s) != hipSuccess)
+    return done(fail(ZP_ERR_DEVICE, "upload failed"));
+  hipLaunchKernelGGL(eq_table_kernel, dim3((uint32_t)((row_len + 255) / 256)), dim3(256), 0, s,
+                     d_rr, col_vars, d_q1, P);
+  hipLaunchKernelGGL(eq_table_kernel, dim3((uint32_t)((num_rows + 255) / 256)), dim3(256), 0, s,
+                     d_rr + 2 * col_vars, row_vars, d_q0, P);
+  const uint32_t dp_log2 = ceil_log2(Dk);
+  const uint32_t red_bits = ceil_log2(row_len) + 1;
+  if (binpoly)
+    hipLaunchKernelGGL(plane_sums_kernel<uint64_t>, dim3((uint32_t)rows), dim3(64 * kSumWaves), 0,
+                       s, (const uint64_t*)d_evals, d_q1, d_planes, row_len, Dk, dp_log2, red_bits,
+                       P);
+  else
+    hipLaunchKernelGGL(plane_sums_kernel<uint32_t>, dim3((uint32_t)rows), dim3(64 * kSumWaves), 0,
+                       s, (const uint32_t*)d_evals, d_q1, d_planes, row_len, Dk, dp_log2, red_bits,
+                       P);
+  hipLaunchKernelGGL(b_kernel, dim3((uint32_t)((num_rows + 255) / 256)), dim3(256), 0, s,
+                     d_planes, d_alm, d_b, num_rows, Dk, batch, P);
+  if (hipGetLastError() != hipSuccess) return done(fail(ZP_ERR_DEVICE, "launch failed"));
+  uint8_t* dn = pin + up1;
+  if (hipMemcpyAsync(dn, d_b, num_rows * sizeof(Fp), hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipMemcpyAsync(dn + num_rows * sizeof(Fp), d_q0, num_rows * sizeof(Fp),
+                     hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(fail(ZP_ERR_DEVICE, "b copy failed"));
+  std::vector<Fp> b(num_rows), q0(num_rows);
+  std::memcpy(b.data(), dn, num_rows * sizeof(Fp));
+  std::memcpy(q0.data(), dn + num_rows * sizeof(Fp), num_rows * sizeof(Fp));
+
+  // Transcript: b, eval = <q_0, b>, s, column indices.
+  ts_write_field_elements(t, mod, b.data(), b.size());
+  Fp ev = {{0, 0, 0, 0}};
+  for (uint64_t j = 0; j < num_rows; j++) ev = fp_add(P, ev, fp_mul(P, q0[j], b[j]));
+  const Fp evc = fp_from_mont(P, ev);
+  std::memcpy(h_eval, evc.w, 32);
+  std::vector<int64_t> co(num_rows * 2);
+  if (num_rows == 1)
+    co[0] = 1, co[1] = 0;
+  else
+    for (uint64_t j = 0; j < num_rows; j++) ts_challenge(t, (uint8_t*)&co[2 * j], 16);
+  // The combined row and the columns are written without being absorbed, so
+  // every index follows from the state after s.
+  std::vector<uint64_t> cols(num_openings);
+  for (uint32_t k = 0; k < num_openings; k++) cols[k] = ts_squeeze_idx(t, N);
+
+  // Round 2: combined row, columns, paths.
+  uint8_t* up = pin;
+  std::memcpy(up, al.data(), al.size() * 8);
+  std::memcpy(up + al.size() * 8, co.data(), num_rows * 16);
+  std::memcpy(up + al.size() * 8 + num_rows * 16, cols.data(), (uint64_t)num_openings * 8);
+  if (hipMemcpyAsync(d_al, up, al.size() * 8, hipMemcpyHostToDevice, s) != hipSuccess ||
+      hipMemcpyAsync(d_co, up + al.size() * 8, num_rows * 16, hipMemcpyHostToDevice, s) !=
+          hipSuccess ||
+      (num_openings && hipMemcpyAsync(d_cols, up + al.size() * 8 + num_rows * 16,
+                                      (uint64_t)num_openings * 8, hipMemcpyHostToDevice,
+                                      s) != hipSuccess))
+    return done(fail(ZP_ERR_DEVICE, "upload failed"));
+  const uint64_t* ev64 = (const uint64_t*)d_evals;
+  if (!binpoly) {
+    const uint64_t n = rows * row_len;
+    hipLaunchKernelGGL(widen_i32_kernel, dim3((uint32_t)((n + 255) / 256)), dim3(256), 0, s,
+                       (const uint32_t*)d_evals, d_wide, n);
+    ev64 = d_wide;
+  }
+  rc = launch_combined_row(Dk, L, num_rows, row_len, batch, ev64, d_al, d_co, d_comb, d_scr, s);
+  if (rc == ZP_OK) rc = launch_gather_openings(hint, d_cols, num_openings, d_columns, d_paths, s);
+  if (rc != ZP_OK) return done(rc);
+  uint8_t* o_comb = pin + up2;
+  uint8_t* o_cols = o_comb + row_len * L * 8;
+  uint8_t* o_paths = o_cols + (uint64_t)num_openings * rows * elem;
+  if (hipMemcpyAsync(o_comb, d_comb, row_len * L * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      (num_openings &&
+       hipMemcpyAsync(o_cols, d_columns, (uint64_t)num_openings * rows * elem,
+                      hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (num_openings && levels &&
+       hipMemcpyAsync(o_paths, d_paths, (uint64_t)num_openings * levels * 32,
+                      hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return done(fail(ZP_ERR_DEVICE, "proof copy failed"));
+
+  // Proof bytes: combined row (write_const_many), then per opening the
+  // column of every poly and write_merkle_proof (leaf, leaf count, path
+  // length as u64, then the siblings bottom-up).
+  ts_put(t, o_comb, row_len * L * 8);
+  for (uint32_t k = 0; k < num_openings; k++) {
+    ts_put(t, o_cols + (uint64_t)k * rows * elem, rows * elem);
+    const uint64_t hdr[3] = {cols[k], N, levels};
+    ts_put(t, hdr, sizeof hdr);
+    ts_put(t, o_paths + (uint64_t)k * levels * 32, (uint64_t)levels * 32);
+  }
+  return done(ZP_OK);
+}

@@ -35,35 +35,6 @@ double us_since(Clock::time_point t0, Clock::time_point t) {
   return std::chrono::duration<double, std::micro>(t - t0).count();
 }
 
-// Column openings: out[(k*rows + r)*words + w] = cw[(r*N + col_k)*words + w].
-__global__ void gather_columns_kernel(const uint32_t* __restrict__ cw,
-                                      const uint64_t* __restrict__ cols, uint32_t* __restrict__ out,
-                                      uint64_t rows, uint64_t N, uint32_t words, uint32_t n_cols) {
-  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t total = (uint64_t)n_cols * rows * words;
-  if (idx >= total) return;
-  const uint32_t w = (uint32_t)(idx % words);
-  const uint64_t r = (idx / words) % rows;
-  const uint64_t k = idx / words / rows;
-  out[idx] = cw[(r * N + cols[k]) * words + w];
-}
-
-// Merkle paths: out[(k*(height-1) + l)*8 + w] = word w of the sibling of
-// column k's ancestor at level l (layers: leaves first, 32 bytes per node).
-__global__ void gather_paths_kernel(const uint32_t* __restrict__ layers,
-                                    const uint64_t* __restrict__ cols, uint32_t* __restrict__ out,
-                                    uint64_t N, uint32_t levels, uint32_t n_cols) {
-  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t total = (uint64_t)n_cols * levels * 8;
-  if (idx >= total) return;
-  const uint32_t w = (uint32_t)(idx % 8);
-  const uint32_t l = (uint32_t)((idx / 8) % levels);
-  const uint64_t k = idx / 8 / levels;
-  const uint64_t off = 2 * N - 2 * (N >> l);  // first node of level l
-  const uint64_t node = (cols[k] >> l) ^ 1;
-  out[idx] = layers[(off + node) * 8 + w];
-}
-
 }  // namespace
 }  // namespace zpi
 
@@ -178,7 +149,7 @@ int run_open(zp_server::Request& r, hipStream_t s) {
   const zp_commit_desc& d = r.desc;
   zp_hint* h = r.hint;
   const uint32_t n_cols = (uint32_t)r.cols.size();
-  const uint64_t rows = h->num_rows * h->batch, words = h->elem / 4;
+  const uint64_t rows = h->num_rows * h->batch;
   const uint32_t levels = h->height - 1;
   std::vector<void*> tmp;
   auto alloc = [&](uint64_t bytes, void** p) {
@@ -218,19 +189,8 @@ int run_open(zp_server::Request& r, hipStream_t s) {
     if (rc == ZP_OK &&
         hipMemcpyAsync(dc, r.cols.data(), n_cols * 8, hipMemcpyHostToDevice, s) != hipSuccess)
       rc = fail(ZP_ERR_DEVICE, "column index upload failed");
-    if (rc == ZP_OK) {
-      const uint64_t tc = n_cols * rows * words;
-      hipLaunchKernelGGL(gather_columns_kernel, dim3((uint32_t)((tc + 255) / 256)), dim3(256),
-                         0, s, (const uint32_t*)h->d_cw, (const uint64_t*)dc,
-                         (uint32_t*)colout, rows, h->N, (uint32_t)words, n_cols);
-      if (levels) {
-        const uint64_t tp = (uint64_t)n_cols * levels * 8;
-        hipLaunchKernelGGL(gather_paths_kernel, dim3((uint32_t)((tp + 255) / 256)), dim3(256),
-                           0, s, (const uint32_t*)h->d_layers, (const uint64_t*)dc,
-                           (uint32_t*)pathout, h->N, levels, n_cols);
-      }
-      if (hipGetLastError() != hipSuccess) rc = fail(ZP_ERR_DEVICE, "gather launch failed");
-    }
+    if (rc == ZP_OK)
+      rc = launch_gather_openings(h, (const uint64_t*)dc, n_cols, colout, pathout, s);
     if (rc == ZP_OK &&
         (hipMemcpyAsync(r.h_columns, colout, n_cols * rows * h->elem, hipMemcpyDeviceToHost,
                         s) != hipSuccess ||

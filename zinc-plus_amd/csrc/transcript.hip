@@ -1,0 +1,352 @@
+// transcript.hip -- the Fiat-Shamir transcript and the proof byte stream
+// (host code).
+//
+// Reference: transcript/src/lib.rs:10-72 (Blake3Transcript: challenges are
+// the XOF output of the running BLAKE3 state, then absorbed between the tag
+// bytes 0x12 and 0x34; get_prime absorbs every candidate),
+// transcript/src/traits.rs:358-420 (get_challenges, get_random_field_cfg,
+// absorb_slice with tags 0x6/0x7, absorb_random_field with tags
+// 0x3 0x5 0x1 0x3), zip-plus/src/pcs_transcript.rs:27-221
+// (PcsProverTranscript: squeeze_challenge_idx, write_field_elements,
+// write_const_many, write_merkle_proof), primality/src/lib.rs:9-20
+// (MillerRabin: the strong probable-prime test to base 2).
+//
+// The hasher is an incremental BLAKE3 (the published specification: 1 KiB
+// chunks, a stack of chunk chaining values merged as chunks complete, the
+// root node recompressed with increasing counters for extended output).
+#include <cstring>
+#include <vector>
+
+#include "common.h"
+#include "field.h"
+
+namespace zpi {
+namespace {
+
+const uint32_t kIV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
+                         0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
+const uint8_t kSchedule[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+enum : uint32_t { kChunkStart = 1, kChunkEnd = 2, kParent = 4, kRoot = 8 };
+
+inline uint32_t rotr(uint32_t x, int r) { return (x >> r) | (x << (32 - r)); }
+
+void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, uint32_t len,
+              uint32_t flags, uint32_t out[16]) {
+  uint32_t v[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7],
+                    kIV[0], kIV[1], kIV[2], kIV[3], (uint32_t)counter, (uint32_t)(counter >> 32),
+                    len, flags};
+  uint32_t m[16];
+  std::memcpy(m, m_in, 64);
+  auto g = [&](int a, int b, int c, int d, uint32_t x, uint32_t y) {
+    v[a] += v[b] + x;
+    v[d] = rotr(v[d] ^ v[a], 16);
+    v[c] += v[d];
+    v[b] = rotr(v[b] ^ v[c], 12);
+    v[a] += v[b] + y;
+    v[d] = rotr(v[d] ^ v[a], 8);
+    v[c] += v[d];
+    v[b] = rotr(v[b] ^ v[c], 7);
+  };
+  for (int r = 0; r < 7; r++) {
+    g(0, 4, 8, 12, m[0], m[1]);
+    g(1, 5, 9, 13, m[2], m[3]);
+    g(2, 6, 10, 14, m[4], m[5]);
+    g(3, 7, 11, 15, m[6], m[7]);
+    g(0, 5, 10, 15, m[8], m[9]);
+    g(1, 6, 11, 12, m[10], m[11]);
+    g(2, 7, 8, 13, m[12], m[13]);
+    g(3, 4, 9, 14, m[14], m[15]);
+    uint32_t t[16];
+    for (int i = 0; i < 16; i++) t[i] = m[kSchedule[i]];
+    std::memcpy(m, t, 64);
+  }
+  for (int i = 0; i < 8; i++) {
+    out[i] = v[i] ^ v[i + 8];
+    out[i + 8] = v[i + 8] ^ cv[i];
+  }
+}
+
+void words_le(const uint8_t* b, uint32_t* w, int n) {
+  for (int i = 0; i < n; i++)
+    w[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) |
+           ((uint32_t)b[4 * i + 3] << 24);
+}
+
+// The node whose compression gives a chaining value or the root output.
+struct Node {
+  uint32_t cv[8];
+  uint32_t block[16];
+  uint64_t counter;
+  uint32_t len, flags;
+  void chaining(uint32_t out[8]) const {
+    uint32_t o[16];
+    compress(cv, block, counter, len, flags, o);
+    std::memcpy(out, o, 32);
+  }
+};
+
+Node parent_node(const uint32_t l[8], const uint32_t r[8]) {
+  Node n;
+  std::memcpy(n.cv, kIV, 32);
+  std::memcpy(n.block, l, 32);
+  std::memcpy(n.block + 8, r, 32);
+  n.counter = 0;
+  n.len = 64;
+  n.flags = kParent;
+  return n;
+}
+
+}  // namespace
+
+class Blake3 {
+ public:
+  Blake3() { reset_chunk(0); }
+
+  void update(const uint8_t* p, size_t n) {
+    while (n) {
+      if (chunk_len() == 1024) {
+        uint32_t cv[8];
+        chunk_node().chaining(cv);
+        push_chunk(cv, chunk_ + 1);
+        reset_chunk(chunk_ + 1);
+      }
+      if (buf_len_ == 64) {  // a full block is compressed only once more input follows
+        uint32_t m[16], o[16];
+        words_le(buf_, m, 16);
+        compress(cv_, m, chunk_, 64, blocks_ == 0 ? kChunkStart : 0, o);
+        std::memcpy(cv_, o, 32);
+        blocks_++;
+        buf_len_ = 0;
+      }
+      const size_t take = std::min<size_t>(n, 64 - buf_len_);
+      std::memcpy(buf_ + buf_len_, p, take);
+      buf_len_ += (uint32_t)take;
+      p += take;
+      n -= take;
+    }
+  }
+
+  // finalize_xof().fill(out): does not change the state.
+  void xof(uint8_t* out, size_t n) const {
+    Node node = chunk_node();
+    for (size_t i = stack_.size(); i-- > 0;) {
+      uint32_t cv[8];
+      node.chaining(cv);
+      node = parent_node(stack_[i].w, cv);
+    }
+    for (uint64_t blk = 0; n; blk++) {
+      uint32_t o[16];
+      compress(node.cv, node.block, blk, node.len, node.flags | kRoot, o);
+      uint8_t bytes[64];
+      for (int i = 0; i < 16; i++)
+        for (int k = 0; k < 4; k++) bytes[4 * i + k] = (uint8_t)(o[i] >> (8 * k));
+      const size_t take = std::min<size_t>(n, 64);
+      std::memcpy(out, bytes, take);
+      out += take;
+      n -= take;
+    }
+  }
+
+ private:
+  struct Cv {
+    uint32_t w[8];
+  };
+  uint32_t cv_[8];
+  uint64_t chunk_ = 0;
+  uint8_t buf_[64];
+  uint32_t buf_len_ = 0, blocks_ = 0;
+  std::vector<Cv> stack_;
+
+  size_t chunk_len() const { return (size_t)blocks_ * 64 + buf_len_; }
+
+  void reset_chunk(uint64_t idx) {
+    std::memcpy(cv_, kIV, 32);
+    chunk_ = idx;
+    buf_len_ = 0;
+    blocks_ = 0;
+  }
+
+  Node chunk_node() const {
+    Node n;
+    std::memcpy(n.cv, cv_, 32);
+    uint8_t b[64] = {0};
+    std::memcpy(b, buf_, buf_len_);
+    words_le(b, n.block, 16);
+    n.counter = chunk_;
+    n.len = buf_len_;
+    n.flags = (blocks_ == 0 ? kChunkStart : 0) | kChunkEnd;
+    return n;
+  }
+
+  // Merge completed subtrees: one per trailing zero bit of the chunk count.
+  void push_chunk(const uint32_t cv[8], uint64_t total) {
+    Cv c;
+    std::memcpy(c.w, cv, 32);
+    while ((total & 1) == 0) {
+      Node n = parent_node(stack_.back().w, c.w);
+      stack_.pop_back();
+      n.chaining(c.w);
+      total >>= 1;
+    }
+    stack_.push_back(c);
+  }
+};
+
+// SPRP test to base 2 of an odd n > 3 (primality/src/lib.rs:13-19).
+bool sprp2(const uint64_t n[4]) {
+  if ((n[0] & 1) == 0) return false;
+  if ((n[1] | n[2] | n[3]) == 0 && n[0] < 5) return n[0] == 3;
+  const FpParams P = fp_params(n);
+  uint64_t d[4] = {n[0] - 1, n[1], n[2], n[3]};  // n odd: no borrow
+  int s = 0;
+  while ((d[0] & 1) == 0) {
+    for (int i = 0; i < 3; i++) d[i] = (d[i] >> 1) | (d[i + 1] << 63);
+    d[3] >>= 1;
+    s++;
+  }
+  const Fp one = fp_one(P);
+  const Fp minus_one = fp_neg(P, one);
+  Fp x = one;  // 2^d, left to right: square, then double for a set bit
+  int top = 255;
+  while (top >= 0 && !((d[top / 64] >> (top % 64)) & 1)) top--;
+  for (int b = top; b >= 0; b--) {
+    x = fp_mul(P, x, x);
+    if ((d[b / 64] >> (b % 64)) & 1) x = fp_add(P, x, x);
+  }
+  auto eq = [](const Fp& a, const Fp& b) { return std::memcmp(a.w, b.w, 32) == 0; };
+  if (eq(x, one) || eq(x, minus_one)) return true;
+  for (int r = 1; r < s; r++) {
+    x = fp_mul(P, x, x);
+    if (eq(x, minus_one)) return true;
+    if (eq(x, one)) return false;
+  }
+  return false;
+}
+
+}  // namespace zpi
+
+struct zp_transcript {
+  zpi::Blake3 fs;
+  std::vector<uint8_t> stream;
+
+  void absorb(const void* p, size_t n) { fs.update((const uint8_t*)p, n); }
+  void absorb_tag(uint8_t t) { fs.update(&t, 1); }
+  void challenge(uint8_t* out, size_t n) {
+    fs.xof(out, n);
+    absorb_tag(0x12);
+    absorb(out, n);
+    absorb_tag(0x34);
+  }
+  void put(const void* p, size_t n) {
+    const uint8_t* b = (const uint8_t*)p;
+    stream.insert(stream.end(), b, b + n);
+  }
+};
+
+namespace zpi {
+
+void ts_challenge(zp_transcript* t, uint8_t* out, size_t n) { t->challenge(out, n); }
+
+uint64_t ts_squeeze_idx(zp_transcript* t, uint64_t cap) {
+  uint8_t b[4];
+  t->challenge(b, 4);
+  const uint32_t v = (uint32_t)b[0] | ((uint32_t)b[1] << 8) | ((uint32_t)b[2] << 16) |
+                     ((uint32_t)b[3] << 24);
+  return v % cap;
+}
+
+// write_field_elements (pcs_transcript.rs:99-142): Uint<4> has no length
+// prefix; each element is absorbed (absorb_random_field) and written as
+// modulus bytes, then inner bytes.
+void ts_write_field_elements(zp_transcript* t, const uint64_t mod[4], const Fp* v, size_t n) {
+  for (size_t i = 0; i < n; i++) {
+    t->absorb_tag(0x3);
+    t->absorb(mod, 32);
+    t->absorb_tag(0x5);
+    t->absorb_tag(0x1);
+    t->absorb(v[i].w, 32);
+    t->absorb_tag(0x3);
+    t->put(mod, 32);
+    t->put(v[i].w, 32);
+  }
+}
+
+void ts_put(zp_transcript* t, const void* p, size_t n) { t->put(p, n); }
+
+}  // namespace zpi
+
+using namespace zpi;
+
+extern "C" {
+
+int zp_transcript_create(zp_transcript** out) {
+  if (!out) return fail(ZP_ERR_INVALID_PARAM, "null output");
+  *out = new zp_transcript();
+  return ZP_OK;
+}
+
+int zp_transcript_clone(const zp_transcript* t, zp_transcript** out) {
+  if (!t || !out) return fail(ZP_ERR_INVALID_PARAM, "null transcript");
+  *out = new zp_transcript(*t);
+  return ZP_OK;
+}
+
+void zp_transcript_destroy(zp_transcript* t) { delete t; }
+
+int zp_transcript_absorb_slice(zp_transcript* t, const uint8_t* bytes, uint64_t len) {
+  if (!t || (len && !bytes)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  t->absorb_tag(0x6);
+  t->absorb(bytes, len);
+  t->absorb_tag(0x7);
+  return ZP_OK;
+}
+
+int zp_transcript_challenge(zp_transcript* t, uint8_t* h_out, uint32_t num_bytes) {
+  if (!t || (num_bytes && !h_out)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  t->challenge(h_out, num_bytes);
+  return ZP_OK;
+}
+
+int zp_transcript_squeeze_idx(zp_transcript* t, uint64_t cap, uint64_t* idx) {
+  if (!t || !idx) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  if (cap == 0) return fail(ZP_ERR_SHAPE, "Challenge cap is zero");
+  *idx = ts_squeeze_idx(t, cap);
+  return ZP_OK;
+}
+
+int zp_transcript_random_field_cfg(zp_transcript* t, uint8_t h_modulus[32]) {
+  if (!t || !h_modulus) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  for (;;) {
+    uint8_t b[32];
+    t->fs.xof(b, 32);  // gen_random: fill, absorb without tags
+    t->absorb(b, 32);
+    uint64_t c[4];
+    std::memcpy(c, b, 32);
+    if ((c[0] | c[1] | c[2] | c[3]) == 0) continue;
+    if ((c[0] & 1) == 0) {  // even: subtract one across the limbs
+      const uint64_t one[4] = {1, 0, 0, 0};
+      fp_sub_raw(c, one);
+    }
+    if (sprp2(c)) {
+      std::memcpy(h_modulus, c, 32);
+      return ZP_OK;
+    }
+  }
+}
+
+int zp_transcript_write(zp_transcript* t, const uint8_t* bytes, uint64_t len) {
+  if (!t || (len && !bytes)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  t->put(bytes, len);
+  return ZP_OK;
+}
+
+uint64_t zp_transcript_stream_len(const zp_transcript* t) { return t ? t->stream.size() : 0; }
+
+int zp_transcript_stream(const zp_transcript* t, uint8_t* h_out, uint64_t cap) {
+  if (!t || (cap && !h_out)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  if (cap < t->stream.size()) return fail(ZP_ERR_SHAPE, "buffer smaller than the stream");
+  std::memcpy(h_out, t->stream.data(), t->stream.size());
+  return ZP_OK;
+}
+
+}  // extern "C"

@@ -35,7 +35,7 @@ __all__ = [
     "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory",
     "ZipTypes", "BinPolyTypes", "IntTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
-    "merkle_root", "lib", "set_option", "BatchCommit", "CommitServer",
+    "merkle_root", "lib", "set_option", "BatchCommit", "CommitServer", "PcsProverTranscript",
 ]
 
 
@@ -373,6 +373,77 @@ def merkle_root(rows, elem_bytes, stream=None):
     return bytes(out)
 
 
+class PcsProverTranscript:
+    """PcsProverTranscript (zip-plus/src/pcs_transcript.rs:40-221): the
+    Fiat-Shamir Blake3Transcript (transcript/src/lib.rs:10-72) and the proof
+    byte stream, held by the library."""
+
+    def __init__(self, _handle=None):
+        self._lib = L.load()
+        if _handle is None:
+            h = ctypes.c_void_p()
+            _check(self._lib.zp_transcript_create(ctypes.byref(h)))
+            _handle = h
+        self._h = _handle
+
+    @classmethod
+    def new_from_commitment(cls, comm):
+        return cls.new_from_commitments([comm])
+
+    @classmethod
+    def new_from_commitments(cls, comms):
+        t = cls()
+        for c in comms:
+            t.absorb_slice(c.root)
+        return t
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._lib.zp_transcript_destroy(self._h)
+            self._h = None
+
+    def clone(self):
+        h = ctypes.c_void_p()
+        _check(self._lib.zp_transcript_clone(self._h, ctypes.byref(h)))
+        return PcsProverTranscript(h)
+
+    def absorb_slice(self, data):
+        data = bytes(data)
+        buf = (ctypes.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
+        _check(self._lib.zp_transcript_absorb_slice(self._h, buf, len(data)))
+
+    def get_challenge(self, num_bytes):
+        """get_challenge::<T> for T of num_bytes bytes: its little-endian bytes."""
+        buf = (ctypes.c_uint8 * max(1, num_bytes))()
+        _check(self._lib.zp_transcript_challenge(self._h, buf, num_bytes))
+        return bytes(buf)[:num_bytes]
+
+    def get_challenge_i128(self):
+        return int.from_bytes(self.get_challenge(16), "little", signed=True)
+
+    def squeeze_challenge_idx(self, cap):
+        out = ctypes.c_uint64()
+        _check(self._lib.zp_transcript_squeeze_idx(self._h, cap, ctypes.byref(out)))
+        return out.value
+
+    def get_random_field_cfg(self):
+        """get_random_field_cfg::<MontyField<4>, Uint<4>, MillerRabin>: the modulus."""
+        buf = (ctypes.c_uint8 * 32)()
+        _check(self._lib.zp_transcript_random_field_cfg(self._h, buf))
+        return int.from_bytes(bytes(buf), "little")
+
+    def write(self, data):
+        data = bytes(data)
+        buf = (ctypes.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
+        _check(self._lib.zp_transcript_write(self._h, buf, len(data)))
+
+    def stream(self):
+        n = self._lib.zp_transcript_stream_len(self._h)
+        buf = (ctypes.c_uint8 * max(1, n))()
+        _check(self._lib.zp_transcript_stream(self._h, buf, n))
+        return bytes(buf)[:n]
+
+
 class ZipPlus:
     """ZipPlus<Zt, Lc> (pcs/structs.rs:66-89, phase_commit.rs, phase_prove.rs)."""
 
@@ -429,6 +500,29 @@ class ZipPlus:
 
     def commit_single(self, pp, poly, stream=None):
         return self.commit(pp, [poly], stream)
+
+    def prove(self, transcript, pp, polys, point, hint, modulus, stream=None):
+        """ZipPlus::prove::<MontyField<4>, _> (phase_prove.rs:85-253).
+
+        point: num_vars Python ints (Pt = i128); modulus: the field config
+        drawn from the transcript.  Appends the proof to the transcript and
+        returns the evaluation <q_0, b> as a canonical int."""
+        t = self._stack(pp, polys)
+        pt = np.zeros((max(1, len(point)), 2), dtype=np.int64)
+        for i, v in enumerate(point):
+            if not -(1 << 127) <= v < (1 << 127):
+                raise ValueError("point coordinates are i128")
+            u = v & ((1 << 128) - 1)
+            pt[i, 0] = np.uint64(u & ((1 << 64) - 1)).view(np.int64)
+            pt[i, 1] = np.uint64(u >> 64).view(np.int64)
+        mod = (ctypes.c_uint8 * 32).from_buffer_copy(int(modulus).to_bytes(32, "little"))
+        ev = (ctypes.c_uint8 * 32)()
+        _check(L.load().zp_prove(pp.linear_code._h, self.types.code, self.types.degree_plus_one,
+                                 self.types.comb_limbs, _dev_ptr(t), pp.num_rows, t.shape[0],
+                                 hint._h, pt.ctypes.data_as(ctypes.c_void_p), len(point), mod,
+                                 self.types.num_column_openings, transcript._h, ev,
+                                 _stream(stream)))
+        return int.from_bytes(bytes(ev), "little")
 
     def combined_row(self, pp, polys, alphas, coeffs, stream=None):
         """The combined row of prove_f (phase_prove.rs:153-244).

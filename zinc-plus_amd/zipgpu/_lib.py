@@ -105,6 +105,18 @@ SIGNATURES = {
     "zp_server_destroy": (None, [_c_void_p]),
     "zp_combined_row": (_int, [_u32, _u32, _u64, _u64, _u32, _c_void_p, _c_void_p, _c_void_p,
                                _c_void_p, _c_void_p]),
+    "zp_transcript_create": (_int, [ctypes.POINTER(_c_void_p)]),
+    "zp_transcript_clone": (_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
+    "zp_transcript_destroy": (None, [_c_void_p]),
+    "zp_transcript_absorb_slice": (_int, [_c_void_p, _c_void_p, _u64]),
+    "zp_transcript_challenge": (_int, [_c_void_p, _c_void_p, _u32]),
+    "zp_transcript_squeeze_idx": (_int, [_c_void_p, _u64, ctypes.POINTER(_u64)]),
+    "zp_transcript_random_field_cfg": (_int, [_c_void_p, _c_void_p]),
+    "zp_transcript_write": (_int, [_c_void_p, _c_void_p, _u64]),
+    "zp_transcript_stream_len": (_u64, [_c_void_p]),
+    "zp_transcript_stream": (_int, [_c_void_p, _c_void_p, _u64]),
+    "zp_prove": (_int, [_c_void_p, _int, _u32, _u32, _c_void_p, _u64, _u32, _c_void_p,
+                        _c_void_p, _u32, _c_void_p, _u32, _c_void_p, _c_void_p, _c_void_p]),
 }
 
 _lib = None
