@@ -173,6 +173,53 @@ def leg_encode_wide(torch, zipgpu, synth, dev):
     return out
 
 
+def leg_prove(torch, zipgpu, synth, dev, cpu=True):
+    """ZipPlus::prove (alphas, b, combined row, 147 column openings, proof
+    bytes) after a commit, at the reference bench's point (all ones)."""
+    import numpy as np
+
+    out = {}
+    for name, kind, n, row_len, d in (("iprs_2p20_d32", "iprs", 1 << 20, 1 << 14, 32),
+                                      ("raa_2p22_d64", "raa", 1 << 22, 1 << 11, 64)):
+        types = zipgpu.BinPolyTypes(d)
+        code = (zipgpu.IprsCode.new_with_optimal_depth(row_len) if kind == "iprs"
+                else zipgpu.RaaCode(row_len))
+        zp = zipgpu.ZipPlus(types)
+        pp = zp.setup(n, code)
+        x = torch.from_numpy(synth.binary_polys(7, n, d)).to(dev)
+        hint, comm = zp.commit(pp, [x])
+        base = zipgpu.PcsProverTranscript.new_from_commitment(comm)
+        p = base.get_random_field_cfg()
+        point = [1] * (n.bit_length() - 1)
+        last = {}
+
+        def step():
+            t = base.clone()
+            last["ev"] = zp.prove(t, pp, [x], point, hint, p)
+            last["t"] = t
+        ms = wall_ms(torch, step, 10, 2)
+        out[name] = {"prove_ms": ms, "proof_bytes": len(last["t"].stream())}
+        hint.free()
+    if cpu:
+        from oracle import oracle
+        threads = min(16, os.cpu_count() or 1)
+        oracle.set_threads(threads)
+        n, row_len, rows = 1 << 20, 1 << 14, 64
+        xs = synth.binary_polys(7, n, 32).view(np.uint64)
+        c = oracle.commit("iprs", xs, rows, row_len, D=32, want_cw=True, want_layers=True)
+        ts = oracle.Transcript()
+        ts.absorb_slice(c["root"])
+        p = ts.random_field_cfg()
+        t0 = time.perf_counter()
+        oracle.prove(ts, p, xs, rows, row_len, 4 * row_len, 256, c["cw"], c["layers"], [1] * 20,
+                     D=32, L=5)
+        out["iprs_2p20_d32"]["cpu_oracle_prove_ms"] = (time.perf_counter() - t0) * 1e3
+        out["iprs_2p20_d32"]["cpu_cores"] = threads
+    out["workload"] = ("prove after commit: 2^20 BinaryPoly<32> IPRS row 2^14 x 64, and 2^22 "
+                       "BinaryPoly<64> RAA row 2^11 x 2^11; MontyField<4>, Int<5>, 147 openings")
+    return out
+
+
 def leg_batched(torch, zipgpu, synth, dev):
     """64 commits, log2 n uniform in 12..20, D 32/64, IPRS/RAA: one batch plan
     (HIP graph, autotuned variants) against one zp_commit per poly."""
@@ -321,6 +368,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-legs", action="store_true", help="skip the other workloads")
+    ap.add_argument("--legs", default="", help="comma-separated workload names to run (default all)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run plain 1 GiB copies (counter calibration)")
     args = ap.parse_args()
@@ -423,7 +471,10 @@ def main():
         legs = {}
         for name, fn in (("plain_zip_2p16_i32", leg_plain_zip), ("raa_2p22_d64", leg_raa),
                          ("batched_64_mixed", leg_batched), ("served_commit_open", leg_served),
-                         ("encode_wide_int5", leg_encode_wide)):
+                         ("encode_wide_int5", leg_encode_wide),
+                         ("prove", lambda *a: leg_prove(*a, cpu=not args.no_cpu))):
+            if args.legs and name not in args.legs.split(","):
+                continue
             log(f"leg {name} ...")
             legs[name] = fn(torch, zipgpu, synth, dev)
             log(f"  {legs[name]}")

@@ -13,7 +13,8 @@
 //     b_j = sum_c alpha_c P_c[j],  P_c[j] = sum_{col: bit c set} q_1[col]:
 //     one workgroup per row sums the Montgomery-form q_1 entries of every
 //     plane exactly (288-bit accumulators) and reduces once;
-//   the combined row (combined.hip) and the column / Merkle-path gathers.
+//   the combined row (combined.hip), and the openings packed in
+//   proof-stream order (columns, Merkle proof headers and paths).
 // The host runs the transcript: alphas, the b elements, eval = <q_0, b>,
 // the coefficients s, the column indices, and the proof bytes.
 #include <cstring>
@@ -28,6 +29,7 @@ void ts_challenge(zp_transcript* t, uint8_t* out, size_t n);
 uint64_t ts_squeeze_idx(zp_transcript* t, uint64_t cap);
 void ts_write_field_elements(zp_transcript* t, const uint64_t mod[4], const Fp* v, size_t n);
 void ts_put(zp_transcript* t, const void* p, size_t n);
+void ts_reserve(zp_transcript* t, size_t more);
 
 namespace {
 
@@ -162,6 +164,37 @@ __global__ void gather_paths_kernel(const uint32_t* __restrict__ layers,
   out[idx] = layers[(off + node) * 8 + w];
 }
 
+// The openings in proof-stream order (phase_prove.rs:282-304 with
+// write_merkle_proof, pcs_transcript.rs:210-221): per opening k, the
+// column of every poly's rows, then leaf index, leaf count and path length
+// as u64, then the sibling path bottom-up.  One thread per output word.
+__global__ void pack_openings_kernel(const uint32_t* __restrict__ cw,
+                                     const uint32_t* __restrict__ layers,
+                                     const uint64_t* __restrict__ cols, uint32_t* __restrict__ out,
+                                     uint64_t rows, uint64_t N, uint32_t words, uint32_t levels,
+                                     uint32_t n_cols) {
+  const uint64_t col_words = rows * words;
+  const uint64_t per = col_words + 6 + (uint64_t)levels * 8;
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= per * n_cols) return;
+  const uint64_t k = idx / per, w = idx % per;
+  const uint64_t col = cols[k];
+  uint32_t v;
+  if (w < col_words) {
+    v = cw[((w / words) * N + col) * words + w % words];
+  } else if (w < col_words + 6) {
+    const uint32_t h = (uint32_t)(w - col_words);
+    const uint64_t f = h < 2 ? col : (h < 4 ? N : (uint64_t)levels);
+    v = (h & 1) ? (uint32_t)(f >> 32) : (uint32_t)f;
+  } else {
+    const uint64_t pw = w - col_words - 6;
+    const uint32_t l = (uint32_t)(pw / 8);
+    const uint64_t off = 2 * N - 2 * (N >> l);  // first node of level l
+    v = layers[(off + ((col >> l) ^ 1)) * 8 + pw % 8];
+  }
+  out[idx] = v;
+}
+
 uint32_t ceil_log2(uint64_t x) {
   uint32_t r = 0;
   while ((1ull << r) < x) r++;
@@ -290,7 +323,7 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   Fp *d_rr, *d_q1, *d_q0, *d_planes, *d_b, *d_alm;
   int64_t *d_al, *d_co;
   uint64_t *d_cols, *d_comb, *d_wide = nullptr;
-  void *d_scr, *d_columns, *d_paths;
+  void *d_scr, *d_open;
   Bufs bufs;
   bufs.add((void**)&d_rr, rr.size() * sizeof(Fp));
   bufs.add((void**)&d_q1, row_len * sizeof(Fp));
@@ -303,8 +336,8 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   bufs.add((void**)&d_cols, (uint64_t)num_openings * 8);
   bufs.add((void**)&d_comb, row_len * L * 8);
   bufs.add(&d_scr, combined_row_scratch_bytes(Dk, row_len, batch, num_rows));
-  bufs.add(&d_columns, (uint64_t)num_openings * rows * elem);
-  bufs.add(&d_paths, (uint64_t)num_openings * levels * 32);
+  const uint64_t open_bytes = rows * elem + 24 + (uint64_t)levels * 32;  // one opening
+  bufs.add(&d_open, (uint64_t)num_openings * open_bytes);
   if (!binpoly) bufs.add((void**)&d_wide, rows * row_len * 8);
   ZP_TRY(bufs.alloc(s));
   int rc = ZP_OK;
@@ -315,8 +348,7 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   const uint64_t up1 = rr.size() * sizeof(Fp) + alm.size() * sizeof(Fp);
   const uint64_t down1 = 2 * num_rows * sizeof(Fp);
   const uint64_t up2 = al.size() * 8 + num_rows * 16 + (uint64_t)num_openings * 8;
-  const uint64_t down2 =
-      row_len * L * 8 + (uint64_t)num_openings * rows * elem + (uint64_t)num_openings * levels * 32;
+  const uint64_t down2 = row_len * L * 8 + (uint64_t)num_openings * open_bytes;
   uint8_t* pin = pinned_scratch(std::max(up1 + down1, up2 + down2));
   if (!pin) return done(fail(ZP_ERR_OOM, "pinned staging allocation failed"));
 
@@ -390,30 +422,24 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
     ev64 = d_wide;
   }
   rc = launch_combined_row(Dk, L, num_rows, row_len, batch, ev64, d_al, d_co, d_comb, d_scr, s);
-  if (rc == ZP_OK) rc = launch_gather_openings(hint, d_cols, num_openings, d_columns, d_paths, s);
   if (rc != ZP_OK) return done(rc);
+  if (num_openings) {
+    const uint64_t tw = (uint64_t)num_openings * open_bytes / 4;
+    hipLaunchKernelGGL(pack_openings_kernel, dim3((uint32_t)((tw + 255) / 256)), dim3(256), 0, s,
+                       (const uint32_t*)hint->d_cw, (const uint32_t*)hint->d_layers, d_cols,
+                       (uint32_t*)d_open, rows, N, (uint32_t)(elem / 4), levels, num_openings);
+    if (hipGetLastError() != hipSuccess) return done(fail(ZP_ERR_DEVICE, "launch failed"));
+  }
   uint8_t* o_comb = pin + up2;
-  uint8_t* o_cols = o_comb + row_len * L * 8;
-  uint8_t* o_paths = o_cols + (uint64_t)num_openings * rows * elem;
+  uint8_t* o_open = o_comb + row_len * L * 8;
   if (hipMemcpyAsync(o_comb, d_comb, row_len * L * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      (num_openings &&
-       hipMemcpyAsync(o_cols, d_columns, (uint64_t)num_openings * rows * elem,
-                      hipMemcpyDeviceToHost, s) != hipSuccess) ||
-      (num_openings && levels &&
-       hipMemcpyAsync(o_paths, d_paths, (uint64_t)num_openings * levels * 32,
-                      hipMemcpyDeviceToHost, s) != hipSuccess) ||
+      (num_openings && hipMemcpyAsync(o_open, d_open, (uint64_t)num_openings * open_bytes,
+                                      hipMemcpyDeviceToHost, s) != hipSuccess) ||
       hipStreamSynchronize(s) != hipSuccess)
     return done(fail(ZP_ERR_DEVICE, "proof copy failed"));
 
-  // Proof bytes: combined row (write_const_many), then per opening the
-  // column of every poly and write_merkle_proof (leaf, leaf count, path
-  // length as u64, then the siblings bottom-up).
-  ts_put(t, o_comb, row_len * L * 8);
-  for (uint32_t k = 0; k < num_openings; k++) {
-    ts_put(t, o_cols + (uint64_t)k * rows * elem, rows * elem);
-    const uint64_t hdr[3] = {cols[k], N, levels};
-    ts_put(t, hdr, sizeof hdr);
-    ts_put(t, o_paths + (uint64_t)k * levels * 32, (uint64_t)levels * 32);
-  }
+  // Proof bytes: the combined row (write_const_many), then the openings.
+  ts_reserve(t, down2);
+  ts_put(t, o_comb, down2);
   return done(ZP_OK);
 }

@@ -14,7 +14,11 @@
 // The hasher is an incremental BLAKE3 (the published specification: 1 KiB
 // chunks, a stack of chunk chaining values merged as chunks complete, the
 // root node recompressed with increasing counters for extended output).
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
+#include <new>
+#include <thread>
 #include <vector>
 
 #include "common.h"
@@ -225,9 +229,66 @@ bool sprp2(const uint64_t n[4]) {
 
 }  // namespace zpi
 
+namespace zpi {
+
+// Copies n bytes; large copies are split over host threads, which also
+// spreads the first-touch page faults of a freshly grown destination.
+void par_memcpy(void* dst, const void* src, size_t n) {
+  const size_t kMin = 8u << 20;
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < kMin || hw == 1) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  const unsigned T = (unsigned)std::min<size_t>(hw, n / (2u << 20));
+  const size_t per = (n / T + 4095) & ~(size_t)4095;
+  std::vector<std::thread> th;
+  for (unsigned i = 1; i < T; i++) {
+    const size_t off = i * per;
+    if (off >= n) break;
+    th.emplace_back([=] {
+      std::memcpy((uint8_t*)dst + off, (const uint8_t*)src + off, std::min(per, n - off));
+    });
+  }
+  std::memcpy(dst, src, std::min(per, n));
+  for (auto& x : th) x.join();
+}
+
+// The proof stream's bytes: grown with realloc (large blocks are remapped,
+// not copied).
+struct ByteBuf {
+  uint8_t* p = nullptr;
+  size_t n = 0, cap = 0;
+  ByteBuf() = default;
+  ByteBuf(const ByteBuf& o) {
+    reserve(o.n);
+    if (o.n) par_memcpy(p, o.p, o.n);
+    n = o.n;
+  }
+  ByteBuf& operator=(const ByteBuf&) = delete;
+  ~ByteBuf() { std::free(p); }
+  void reserve(size_t want) {
+    if (want <= cap) return;
+    const size_t c = std::max(want, cap * 2);
+    uint8_t* q = (uint8_t*)std::realloc(p, std::max<size_t>(c, 64));
+    if (!q) throw std::bad_alloc();
+    p = q;
+    cap = std::max<size_t>(c, 64);
+  }
+  // Appends k uninitialised bytes and returns where they start.
+  uint8_t* grow(size_t k) {
+    reserve(n + k);
+    uint8_t* at = p + n;
+    n += k;
+    return at;
+  }
+};
+
+}  // namespace zpi
+
 struct zp_transcript {
   zpi::Blake3 fs;
-  std::vector<uint8_t> stream;
+  zpi::ByteBuf stream;
 
   void absorb(const void* p, size_t n) { fs.update((const uint8_t*)p, n); }
   void absorb_tag(uint8_t t) { fs.update(&t, 1); }
@@ -238,8 +299,7 @@ struct zp_transcript {
     absorb_tag(0x34);
   }
   void put(const void* p, size_t n) {
-    const uint8_t* b = (const uint8_t*)p;
-    stream.insert(stream.end(), b, b + n);
+    if (n) zpi::par_memcpy(stream.grow(n), p, n);
   }
 };
 
@@ -273,6 +333,8 @@ void ts_write_field_elements(zp_transcript* t, const uint64_t mod[4], const Fp* 
 
 void ts_put(zp_transcript* t, const void* p, size_t n) { t->put(p, n); }
 
+void ts_reserve(zp_transcript* t, size_t more) { t->stream.reserve(t->stream.n + more); }
+
 }  // namespace zpi
 
 using namespace zpi;
@@ -287,7 +349,11 @@ int zp_transcript_create(zp_transcript** out) {
 
 int zp_transcript_clone(const zp_transcript* t, zp_transcript** out) {
   if (!t || !out) return fail(ZP_ERR_INVALID_PARAM, "null transcript");
-  *out = new zp_transcript(*t);
+  try {
+    *out = new zp_transcript(*t);
+  } catch (const std::bad_alloc&) {
+    return fail(ZP_ERR_OOM, "host allocation failed");
+  }
   return ZP_OK;
 }
 
@@ -340,12 +406,12 @@ int zp_transcript_write(zp_transcript* t, const uint8_t* bytes, uint64_t len) {
   return ZP_OK;
 }
 
-uint64_t zp_transcript_stream_len(const zp_transcript* t) { return t ? t->stream.size() : 0; }
+uint64_t zp_transcript_stream_len(const zp_transcript* t) { return t ? t->stream.n : 0; }
 
 int zp_transcript_stream(const zp_transcript* t, uint8_t* h_out, uint64_t cap) {
   if (!t || (cap && !h_out)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
-  if (cap < t->stream.size()) return fail(ZP_ERR_SHAPE, "buffer smaller than the stream");
-  std::memcpy(h_out, t->stream.data(), t->stream.size());
+  if (cap < t->stream.n) return fail(ZP_ERR_SHAPE, "buffer smaller than the stream");
+  if (t->stream.n) par_memcpy(h_out, t->stream.p, t->stream.n);
   return ZP_OK;
 }
 
