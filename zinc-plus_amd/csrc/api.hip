@@ -46,6 +46,36 @@ int upload(const std::vector<T>& host, T** dev) {
   return ZP_OK;
 }
 
+// Butterfly coefficients T(j, t) = tw[(j-1)*8 + (t*j mod 8)] of each
+// position (butterfly.rs:24-43).  The base twiddles satisfy B[u+4] = -B[u],
+// so T(j, t+4) = T(j, t) for even j and, for odd j, the normalised negation
+// of T(j, t): -T(j, t), or -T(j, t) - p where the negation leaves
+// [-32769, 32767].  Keeps T(j, t) for t < 4 and marks the -p cases.
+int symmetric_twiddles(const std::vector<int32_t>& tw, std::vector<double>& out,
+                       std::vector<uint16_t>& mask) {
+  const int64_t p = 65537;
+  const uint64_t npos = tw.size() / 56;
+  out.assign(npos * 32, 0.0);
+  mask.assign(npos, 0);
+  for (uint64_t i = 0; i < npos; i++) {
+    const int32_t* row = &tw[i * 56];
+    auto T = [&](int j, int t) { return (int64_t)row[(j - 1) * 8 + ((t * j) & 7)]; };
+    for (int j = 1; j <= 7; j++)
+      for (int t = 0; t < 4; t++) {
+        out[i * 32 + (j - 1) * 4 + t] = (double)T(j, t);
+        const int64_t hi = T(j, t + 4);
+        if (j % 2 == 0) {
+          if (hi != T(j, t)) return fail(ZP_ERR_INVALID_PARAM, "twiddle table is not symmetric");
+        } else if (hi == -T(j, t) - p) {
+          mask[i] |= (uint16_t)(1u << (((j - 1) / 2) * 4 + t));
+        } else if (hi != -T(j, t)) {
+          return fail(ZP_ERR_INVALID_PARAM, "twiddle table is not symmetric");
+        }
+      }
+  }
+  return ZP_OK;
+}
+
 void free_code(zp_code* c) {
   if (!c) return;
   (void)hipFree(c->d_ftab);
@@ -53,6 +83,8 @@ void free_code(zp_code* c) {
   (void)hipFree(c->d_afrag);
   (void)hipFree(c->d_tw0f);
   (void)hipFree(c->d_tw1f);
+  (void)hipFree(c->d_q0f);
+  (void)hipFree(c->d_q1f);
   (void)hipFree(c->d_perm1);
   (void)hipFree(c->d_perm2);
   delete c;
@@ -268,11 +300,12 @@ int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_c
         std::vector<int8_t> af((uint64_t)(bd / 16) * ksteps * 3 * 64 * 16);
         iprs_afrag(bd, bl, ksteps, f.data(), af.data());
         rc = upload(af, (int8_t**)&c->d_afrag);
-        std::vector<double> t0(tws[0].begin(), tws[0].end());
-        if (rc == ZP_OK) rc = upload(t0, &c->d_tw0f);
-        if (rc == ZP_OK && c->depth == 2) {
-          std::vector<double> t1(tws[1].begin(), tws[1].end());
-          rc = upload(t1, &c->d_tw1f);
+        for (uint32_t k = 0; rc == ZP_OK && k < c->depth; k++) {
+          std::vector<double> tf;
+          std::vector<uint16_t> qm;
+          rc = symmetric_twiddles(tws[k], tf, qm);
+          if (rc == ZP_OK) rc = upload(tf, k == 0 ? &c->d_tw0f : &c->d_tw1f);
+          if (rc == ZP_OK) rc = upload(qm, k == 0 ? &c->d_q0f : &c->d_q1f);
         }
         if (rc == ZP_OK) c->mfma_ksteps = ksteps;
       }

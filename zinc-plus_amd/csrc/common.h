@@ -46,8 +46,15 @@ struct zp_code {
   // MFMA encoder tables (depth 1 or 2): int8 slices of the base matrix in
   // the v_mfma_i32_16x16x64_i8 operand layout, and stage twiddles as double.
   void* d_afrag = nullptr;    // [bd/16][ksteps][3 slices][64 lanes][16 bytes]
-  double* d_tw0f = nullptr;   // stage 0, [bd][7][8]
-  double* d_tw1f = nullptr;   // stage 1, [8*bd][7][8] (depth 2)
+  // Stage twiddles as double in butterfly order: entry (j-1)*4 + t of a
+  // position holds the coefficient of input j in output t < 4 (outputs
+  // t + 4 follow by symmetry); 32 entries per position, 28 used.
+  double* d_tw0f = nullptr;   // stage 0, [bd][32]
+  double* d_tw1f = nullptr;   // stage 1, [8*bd][32] (depth 2)
+  // Per position, bit ((j-1)/2)*4 + t set where the odd-j coefficient of
+  // output t + 4 is -(coefficient of t) - p instead of its negation.
+  uint16_t* d_q0f = nullptr;  // stage 0, [bd]
+  uint16_t* d_q1f = nullptr;  // stage 1, [8*bd]
   uint32_t mfma_ksteps = 0;   // ceil(bl / 64)
   // RAA
   uint32_t* d_perm1 = nullptr;

@@ -13,10 +13,13 @@
 //
 // The radix-8 stages run in FP64 FMA.  With |B| < 2^23 and |tw| <= 32769,
 // stage-0 products and sums stay below 2^42, so they are exact.  Stage-1
-// inputs (< 2^42) are split at 2^21 so every product and partial sum stays
-// below 2^40, again exact; the two halves are recombined in int64.  The
-// results equal the reference's wrapping-i64 arithmetic bit for bit because
-// no value on this path exceeds 2^59.
+// inputs (< 2^42) are split at 2^32 so every product and partial sum stays
+// below 2^51, again exact; the two halves are recombined in int64.  Each
+// butterfly is evaluated as even and odd half-sums (the base twiddles
+// satisfy B[u+4] = -B[u]), with an exact correction where a coefficient's
+// normalisation differs from its negation by p.  The results equal the
+// reference's wrapping-i64 arithmetic bit for bit because no value on this
+// path exceeds 2^59.
 //
 // Work split: a pre-pass writes, per row and chunk c, the words P[c][w][p]
 // holding the bits of plane p for k = 16w .. 16w+15.  The main kernel owns
@@ -94,111 +97,78 @@ bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ plane
   }
 }
 
-// Radix-8 butterfly steps with the twiddles spread over each 16-lane row of
-// a wave (the lanes of a row share one base row r): lane q of the row holds
-// entries q, q+16, q+32, q+48 (< 56) of that r's 56 twiddles in tw[0..3], and
-// each FMA reads the entry it needs from lane k of its row with a
-// row_newbcast DPP source (v_fmac_f64_dpp), so no lane holds or loads more
-// than 4 of them.  Step jj (1..7) of output t uses entry (jj-1)*8 + (t*jj & 7),
-// i.e. register (jj-1)/2 at lane L(jj, t) = ((jj-1)*8 & 15) + (t*jj & 7).
-//
-// The compiler does not look inside inline asm, so each block starts with the
-// two wait states a DPP read needs after a VALU write of its source.
-template <int JJ, int T>
-constexpr int bcast_lane() {
-  return (((JJ - 1) * 8) & 15) + ((T * JJ) & 7);
+// Radix-8 butterflies by symmetry.  Output t of a butterfly is
+// sum_j x_j T(j, t) with T(j, t+4) = T(j, t) for even j and -T(j, t) for odd
+// j (up to a rare -p, corrected separately), so with
+//   E_t = x_0 + sum_{j even} x_j T(j, t),  O_t = sum_{j odd} x_j T(j, t)
+// the outputs are E_t + O_t and E_t - O_t for t < 4: 28 products and 8 adds
+// instead of 56 products.  The 28 coefficients T(j, t), t < 4, of one
+// position are spread over the 16 lanes of a wave row (lanes of a row share
+// the position): entry e = (j-1)*4 + t sits on lane e % 16 of register
+// e / 16, and each FMA reads it with a row_newbcast DPP source
+// (v_fmac_f64_dpp).  Each asm block starts with the two wait states a DPP
+// read needs after a VALU write of its source.
+template <int J, int T>
+constexpr int sym_lane() {
+  return ((J - 1) * 4 + T) & 15;
 }
 
-// acc[t] += x * tw(lane L(JJ, t)) for t = 0..7, and the same for acc2 with x2.
-template <int JJ>
-__device__ __forceinline__ void bfly16(double (&acc)[8], double (&acc2)[8], double tw, double x,
-                                       double x2) {
-  asm("s_nop 1\n"
-      "v_fmac_f64_dpp %0, %16, %17 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %8, %16, %18 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %1, %16, %17 row_newbcast:%20 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %9, %16, %18 row_newbcast:%20 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %2, %16, %17 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %10, %16, %18 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %3, %16, %17 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %11, %16, %18 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %4, %16, %17 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %12, %16, %18 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %5, %16, %17 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %13, %16, %18 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %6, %16, %17 row_newbcast:%25 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %14, %16, %18 row_newbcast:%25 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %7, %16, %17 row_newbcast:%26 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %15, %16, %18 row_newbcast:%26 row_mask:0xf bank_mask:0xf\n"
-      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]),
-        "+v"(acc[6]), "+v"(acc[7]), "+v"(acc2[0]), "+v"(acc2[1]), "+v"(acc2[2]), "+v"(acc2[3]),
-        "+v"(acc2[4]), "+v"(acc2[5]), "+v"(acc2[6]), "+v"(acc2[7])
-      : "v"(tw), "v"(x), "v"(x2), "i"(bcast_lane<JJ, 0>()), "i"(bcast_lane<JJ, 1>()),
-        "i"(bcast_lane<JJ, 2>()), "i"(bcast_lane<JJ, 3>()), "i"(bcast_lane<JJ, 4>()),
-        "i"(bcast_lane<JJ, 5>()), "i"(bcast_lane<JJ, 6>()), "i"(bcast_lane<JJ, 7>()));
-}
-
-// Steps JJ and JJ+1 (same twiddle register) on one set of accumulators:
-// acc[t] += x * tw(L(JJ, t)), then acc[t] += x2 * tw(L(JJ+1, t)).
-template <int JJ>
-__device__ __forceinline__ void bfly_pair(double (&acc)[8], double tw, double x, double x2) {
+// a[t] += xa * T(JA, t) and b[t] += xb * T(JB, t), t < 4.
+template <int JA, int JB>
+__device__ __forceinline__ void pair4(double (&a)[4], double (&b)[4], double tw, double xa,
+                                      double xb) {
   asm("s_nop 1\n"
       "v_fmac_f64_dpp %0, %8, %9 row_newbcast:%11 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %4, %8, %10 row_newbcast:%15 row_mask:0xf bank_mask:0xf\n"
       "v_fmac_f64_dpp %1, %8, %9 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %5, %8, %10 row_newbcast:%16 row_mask:0xf bank_mask:0xf\n"
       "v_fmac_f64_dpp %2, %8, %9 row_newbcast:%13 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %6, %8, %10 row_newbcast:%17 row_mask:0xf bank_mask:0xf\n"
       "v_fmac_f64_dpp %3, %8, %9 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %4, %8, %9 row_newbcast:%15 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %5, %8, %9 row_newbcast:%16 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %6, %8, %9 row_newbcast:%17 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %7, %8, %9 row_newbcast:%18 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %0, %8, %10 row_newbcast:%19 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %1, %8, %10 row_newbcast:%20 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %2, %8, %10 row_newbcast:%21 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %3, %8, %10 row_newbcast:%22 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %4, %8, %10 row_newbcast:%23 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %5, %8, %10 row_newbcast:%24 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %6, %8, %10 row_newbcast:%25 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %7, %8, %10 row_newbcast:%26 row_mask:0xf bank_mask:0xf\n"
-      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]),
-        "+v"(acc[6]), "+v"(acc[7])
-      : "v"(tw), "v"(x), "v"(x2), "i"(bcast_lane<JJ, 0>()), "i"(bcast_lane<JJ, 1>()),
-        "i"(bcast_lane<JJ, 2>()), "i"(bcast_lane<JJ, 3>()), "i"(bcast_lane<JJ, 4>()),
-        "i"(bcast_lane<JJ, 5>()), "i"(bcast_lane<JJ, 6>()), "i"(bcast_lane<JJ, 7>()),
-        "i"(bcast_lane<JJ + 1, 0>()), "i"(bcast_lane<JJ + 1, 1>()), "i"(bcast_lane<JJ + 1, 2>()),
-        "i"(bcast_lane<JJ + 1, 3>()), "i"(bcast_lane<JJ + 1, 4>()), "i"(bcast_lane<JJ + 1, 5>()),
-        "i"(bcast_lane<JJ + 1, 6>()), "i"(bcast_lane<JJ + 1, 7>()));
+      "v_fmac_f64_dpp %7, %8, %10 row_newbcast:%18 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3]), "+v"(b[0]), "+v"(b[1]), "+v"(b[2]),
+        "+v"(b[3])
+      : "v"(tw), "v"(xa), "v"(xb), "i"(sym_lane<JA, 0>()), "i"(sym_lane<JA, 1>()),
+        "i"(sym_lane<JA, 2>()), "i"(sym_lane<JA, 3>()), "i"(sym_lane<JB, 0>()),
+        "i"(sym_lane<JB, 1>()), "i"(sym_lane<JB, 2>()), "i"(sym_lane<JB, 3>()));
 }
 
-// Step 7 alone.
-__device__ __forceinline__ void bfly_last(double (&acc)[8], double tw, double x) {
+// a[t] += x * T(J, t), t < 4.
+template <int J>
+__device__ __forceinline__ void single4(double (&a)[4], double tw, double x) {
   asm("s_nop 1\n"
-      "v_fmac_f64_dpp %0, %8, %9 row_newbcast:%10 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %1, %8, %9 row_newbcast:%11 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %2, %8, %9 row_newbcast:%12 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %3, %8, %9 row_newbcast:%13 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %4, %8, %9 row_newbcast:%14 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %5, %8, %9 row_newbcast:%15 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %6, %8, %9 row_newbcast:%16 row_mask:0xf bank_mask:0xf\n"
-      "v_fmac_f64_dpp %7, %8, %9 row_newbcast:%17 row_mask:0xf bank_mask:0xf\n"
-      : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3]), "+v"(acc[4]), "+v"(acc[5]),
-        "+v"(acc[6]), "+v"(acc[7])
-      : "v"(tw), "v"(x), "i"(bcast_lane<7, 0>()), "i"(bcast_lane<7, 1>()),
-        "i"(bcast_lane<7, 2>()), "i"(bcast_lane<7, 3>()), "i"(bcast_lane<7, 4>()),
-        "i"(bcast_lane<7, 5>()), "i"(bcast_lane<7, 6>()), "i"(bcast_lane<7, 7>()));
+      "v_fmac_f64_dpp %0, %4, %5 row_newbcast:%6 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %1, %4, %5 row_newbcast:%7 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %2, %4, %5 row_newbcast:%8 row_mask:0xf bank_mask:0xf\n"
+      "v_fmac_f64_dpp %3, %4, %5 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n"
+      : "+v"(a[0]), "+v"(a[1]), "+v"(a[2]), "+v"(a[3])
+      : "v"(tw), "v"(x), "i"(sym_lane<J, 0>()), "i"(sym_lane<J, 1>()), "i"(sym_lane<J, 2>()),
+        "i"(sym_lane<J, 3>()));
 }
 
-// The 56 twiddles of one r (a table row at `row`), spread over the 16 lanes
-// of this lane's row: entry 16m + q on lane q (the address is clamped for the
-// lanes past entry 55 of m = 3, whose value is never read).
-__device__ __forceinline__ void load_spread(double (&tw)[4], const double* row, uint32_t q) {
+// The 28 coefficients of one position (a table row of 32 at `row`): entry
+// 16m + q on lane q of this lane's row.
+__device__ __forceinline__ void load_spread(double (&tw)[2], const double* row, uint32_t q) {
+  tw[0] = row[q];
+  tw[1] = row[16 + q];
+}
+
+// out[t + 4] -= p * x_j for the odd j and t < 4 marked in `mask`
+// (bit ((j-1)/2)*4 + t): the coefficient of output t + 4 there is
+// -T(j, t) - p.
+__device__ __forceinline__ void fix_odd(double (&o)[8], uint32_t mask, const double (&x)[8]) {
 #pragma unroll
-  for (int m = 0; m < 4; m++) tw[m] = row[min(16u * m + q, 55u)];
+  for (int jo = 0; jo < 4; jo++)
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+      if ((mask >> (jo * 4 + t)) & 1u) o[t + 4] = __builtin_fma(-65537.0, x[2 * jo + 1], o[t + 4]);
 }
 
 template <int NC, int S>
 __global__ void __launch_bounds__(256, 2)
 iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ afrag,
                  const double* __restrict__ tw0, const double* __restrict__ tw1,
+                 const uint16_t* __restrict__ q0, const uint16_t* __restrict__ q1,
                  int64_t* __restrict__ out, uint32_t bd, uint32_t D, uint32_t nrb, uint32_t npg,
                  uint64_t row0) {
   // Base sums [c][r_local][p], from the matrix cores to the stage threads.
@@ -249,8 +219,9 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   for (int s = 0; s < S; s++)
 #pragma unroll
     for (int q = 0; q < 3; q++) A[s][q] = afrag[(((uint64_t)rb * S + s) * 3 + q) * 64 + lane];
-  double t0r[4];
-  load_spread(t0r, tw0 + (uint64_t)(rb * 16 + (tid >> 4)) * 56, tid & 15);
+  double t0r[2];
+  load_spread(t0r, tw0 + (uint64_t)(rb * 16 + (tid >> 4)) * 32, tid & 15);
+  const uint32_t m0 = q0[rb * 16 + (tid >> 4)];
   // Keep the loads here: left to the scheduler they sink to their first
   // uses and their latency is exposed there.
   __builtin_amdgcn_sched_barrier(0);
@@ -294,9 +265,11 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   const uint32_t cstride = bd * D * 8;               // byte stride between chunks
 
   // Stage 1's first twiddles are loaded before stage 0, which hides them.
-  double t1n[4];
+  double t1n[2];
+  uint32_t m1n = 0;
   if constexpr (NC == 64) {
-    load_spread(t1n, tw1 + (uint64_t)r * 56, p);
+    load_spread(t1n, tw1 + (uint64_t)r * 32, p);
+    m1n = q1[r];
     __builtin_amdgcn_sched_barrier(0);
   }
 
@@ -307,15 +280,25 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
     double b[8];
 #pragma unroll
     for (int jj = 0; jj < 8; jj++) b[jj] = (double)Bs[((8 * j + jj) * 16 + rl) * 16 + p];
-    double acc[8];
+    double e[4], o[4];
 #pragma unroll
-    for (int t = 0; t < 8; t++) acc[t] = b[0];
-    bfly_pair<1>(acc, t0r[0], b[1], b[2]);
-    bfly_pair<3>(acc, t0r[1], b[3], b[4]);
-    bfly_pair<5>(acc, t0r[2], b[5], b[6]);
-    bfly_last(acc, t0r[3], b[7]);
+    for (int t = 0; t < 4; t++) {
+      e[t] = b[0];
+      o[t] = 0.0;
+    }
+    pair4<2, 1>(e, o, t0r[0], b[2], b[1]);
+    pair4<4, 3>(e, o, t0r[0], b[4], b[3]);
+    pair4<6, 5>(e, o, t0r[1], b[6], b[5]);
+    single4<7>(o, t0r[1], b[7]);
+    double y[8];
 #pragma unroll
-    for (int t = 0; t < 8; t++) y0[8 * j + t] = acc[t];
+    for (int t = 0; t < 4; t++) {
+      y[t] = e[t] + o[t];
+      y[t + 4] = e[t] - o[t];
+    }
+    if (__builtin_expect(m0 != 0, 0)) fix_odd(y, m0, b);
+#pragma unroll
+    for (int t = 0; t < 8; t++) y0[8 * j + t] = y[t];
   }
 
   if constexpr (NC == 8) {
@@ -330,11 +313,13 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
     const double two32 = 4294967296.0, inv32 = 1.0 / 4294967296.0;
 #pragma unroll
     for (int cl = 0; cl < 8; cl++) {
-      double t1r[4];
-#pragma unroll
-      for (int m = 0; m < 4; m++) t1r[m] = t1n[m];
+      double t1r[2];
+      t1r[0] = t1n[0];
+      t1r[1] = t1n[1];
+      const uint32_t m1 = m1n;
       if (cl < 7) {
-        load_spread(t1n, tw1 + ((uint64_t)(cl + 1) * bd + r) * 56, p);
+        load_spread(t1n, tw1 + ((uint64_t)(cl + 1) * bd + r) * 32, p);
+        m1n = q1[(cl + 1) * bd + r];
         __builtin_amdgcn_sched_barrier(0);
       }
       double yh[8], yl[8];
@@ -344,21 +329,35 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
         yh[j] = __builtin_floor(v * inv32);
         yl[j] = __builtin_fma(yh[j], -two32, v);
       }
-      // Sixteen independent chains: high and low halves of the 8 outputs.
-      double sh[8], sl[8];
+      // Even and odd half-sums of the high and low halves.
+      double eh[4], el[4], oh[4], ol[4];
       const double h0 = yh[0] + kMagicHi, l0 = yl[0] + kMagic;
 #pragma unroll
-      for (int t = 0; t < 8; t++) {
-        sh[t] = h0;
-        sl[t] = l0;
+      for (int t = 0; t < 4; t++) {
+        eh[t] = h0;
+        el[t] = l0;
+        oh[t] = 0.0;
+        ol[t] = 0.0;
       }
-      bfly16<1>(sh, sl, t1r[0], yh[1], yl[1]);
-      bfly16<2>(sh, sl, t1r[0], yh[2], yl[2]);
-      bfly16<3>(sh, sl, t1r[1], yh[3], yl[3]);
-      bfly16<4>(sh, sl, t1r[1], yh[4], yl[4]);
-      bfly16<5>(sh, sl, t1r[2], yh[5], yl[5]);
-      bfly16<6>(sh, sl, t1r[2], yh[6], yl[6]);
-      bfly16<7>(sh, sl, t1r[3], yh[7], yl[7]);
+      pair4<2, 2>(eh, el, t1r[0], yh[2], yl[2]);
+      pair4<1, 1>(oh, ol, t1r[0], yh[1], yl[1]);
+      pair4<4, 4>(eh, el, t1r[0], yh[4], yl[4]);
+      pair4<3, 3>(oh, ol, t1r[0], yh[3], yl[3]);
+      pair4<6, 6>(eh, el, t1r[1], yh[6], yl[6]);
+      pair4<5, 5>(oh, ol, t1r[1], yh[5], yl[5]);
+      pair4<7, 7>(oh, ol, t1r[1], yh[7], yl[7]);
+      double sh[8], sl[8];
+#pragma unroll
+      for (int t = 0; t < 4; t++) {
+        sh[t] = eh[t] + oh[t];
+        sh[t + 4] = eh[t] - oh[t];
+        sl[t] = el[t] + ol[t];
+        sl[t + 4] = el[t] - ol[t];
+      }
+      if (__builtin_expect(m1 != 0, 0)) {
+        fix_odd(sh, m1, yh);
+        fix_odd(sl, m1, yl);
+      }
       uint32_t ooff = obase + (uint32_t)cl * cstride;  // output chunk 8t + cl
 #pragma unroll
       for (int t = 0; t < 8; t++) {
@@ -381,7 +380,7 @@ int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32
     const uint64_t nr = std::min(max_rows, rows - r0);
     hipLaunchKernelGGL((iprs_mfma_kernel<NC, S>), dim3((uint32_t)(nr * per_row)), dim3(256), 0, s,
                        planes + r0 * NC * 4 * S * D, (const v4i*)cd->d_afrag, cd->d_tw0f,
-                       cd->d_tw1f, out, cd->bd, D, nrb, npg, r0);
+                       cd->d_tw1f, cd->d_q0f, cd->d_q1f, out, cd->bd, D, nrb, npg, r0);
     ZP_HIP(hipGetLastError());
   }
   return ZP_OK;
