@@ -71,30 +71,44 @@ static_assert((kMagicBits & 0xffffffffull) == 0, "bias must be a multiple of 2^3
 // k = 16w .. 16w+15 in "spread" order: bit i = 4v + b of the group sits at
 // position 8b + v, so (word >> v) & 0x01010101 is operand register v of the
 // MFMA B fragment (bytes b = 0..3 for k = 16w + 4v + b) in two instructions.
+// One thread per (row, c, w, 4 planes), planes fastest: a lane writes its 4
+// words as one 16-byte store (the 8 lanes of a (c, w) cover 128 contiguous
+// bytes for D = 32), and the chunks whose octet reversals are adjacent
+// follow, so one load instruction reads adjacent evals.
 template <int NC>
 __global__ void __launch_bounds__(256)
 bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ planes,
                 uint32_t row_len, uint32_t bl, uint32_t W, uint32_t D, uint32_t depth,
-                uint64_t row0, uint64_t rows) {
+                uint64_t rows) {
+  const uint32_t PQ = D / 4;
   const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rows * W * NC) return;
-  const uint32_t c = (uint32_t)(idx % NC);
-  const uint32_t w = (uint32_t)((idx / NC) % W);
-  const uint64_t row = idx / ((uint64_t)NC * W);
-  const uint64_t* x = evals + (row0 + row) * row_len + octrev_d(c, depth);
+  if (idx >= rows * NC * W * PQ) return;
+  const uint32_t pq = (uint32_t)(idx % PQ);
+  uint64_t rest = idx / PQ;
+  const uint32_t h = (uint32_t)(rest % 8);
+  rest /= 8;
+  const uint32_t a = (uint32_t)(rest % (NC / 8));
+  rest /= NC / 8;
+  const uint32_t w = (uint32_t)(rest % W);
+  const uint64_t row = rest / W;
+  // Depth 2: chunk c = a + 8h has octet reversal 8a + h.  Depth 1: c = h.
+  const uint32_t c = NC == 8 ? h : a + 8 * h;
+  const uint64_t* x = evals + row * row_len + octrev_d(c, depth);
+  // Every load is issued (clamped to the last k) and the values past the
+  // base length are masked: no branch keeps the 16 loads from overlapping.
   uint64_t xs[16];
+#pragma unroll
+  for (int i = 0; i < 16; i++) xs[i] = x[(uint64_t)NC * min(16 * w + i, bl - 1)];
+  uint32_t v[4] = {0, 0, 0, 0};
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const uint32_t k = 16 * w + i;
-    xs[i] = k < bl ? x[(uint64_t)NC * k] : 0ull;
-  }
-  uint32_t* dst = planes + ((row * NC + c) * W + w) * D;
-  for (uint32_t p = 0; p < D; p++) {
-    uint32_t v = 0;
+    const uint32_t bits = k < bl ? (uint32_t)(xs[i] >> (4 * pq)) : 0u;
+    const uint32_t pos = 8 * (i & 3) + (i >> 2);
 #pragma unroll
-    for (int i = 0; i < 16; i++) v |= (uint32_t)((xs[i] >> p) & 1ull) << (8 * (i & 3) + (i >> 2));
-    dst[p] = v;
+    for (int q = 0; q < 4; q++) v[q] |= ((bits >> q) & 1u) << pos;
   }
+  *(uint4*)(planes + ((row * NC + c) * W + w) * D + 4 * pq) = make_uint4(v[0], v[1], v[2], v[3]);
 }
 
 // Radix-8 butterflies by symmetry.  Output t of a butterfly is
@@ -393,10 +407,9 @@ int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t 
   const uint64_t bytes = rows * NC * W * D * sizeof(uint32_t);
   uint32_t* planes = nullptr;
   ZP_TRY(dev_alloc((void**)&planes, bytes, s));
-  const uint64_t threads = rows * W * NC;
+  const uint64_t threads = rows * NC * W * (D / 4);
   hipLaunchKernelGGL((bitplane_kernel<NC>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,
-                     s, evals, planes, (uint32_t)cd->row_len, cd->bl, W, D, cd->depth, 0ull,
-                     rows);
+                     s, evals, planes, (uint32_t)cd->row_len, cd->bl, W, D, cd->depth, rows);
   int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "bitplane launch");
   if (rc == ZP_OK) {
     switch (S) {
