@@ -213,13 +213,14 @@ typedef struct {
 
 typedef struct {
   uint64_t request;
-  int32_t kind;   /* 0 commit finished (root valid), 1 opening finished */
+  int32_t kind;   /* 0 commit finished (root valid), 1 opening finished, 2 proof finished */
   int32_t status; /* zp_status of the job; a failed job ends the request */
   uint8_t root[32];
   double queued_us;  /* arrival (submission) to start on a slot */
   double service_us; /* start to results on the host */
   double latency_us; /* arrival to results on the host */
   char message[128]; /* zp_last_error of a failed job */
+  uint64_t proof_bytes; /* kind 2: length of the proof (zp_server_take_proof) */
 } zp_server_event;
 
 int zp_server_create(const zp_server_config* cfg, zp_server** out);
@@ -242,6 +243,21 @@ int zp_server_open_sizes(zp_server* server, uint64_t request, uint32_t comb_limb
 int zp_server_open(zp_server* server, uint64_t request, uint32_t comb_limbs,
                    const int64_t* h_alphas, const int64_t* h_coeffs, const uint64_t* h_cols,
                    uint32_t n_cols, uint64_t* h_comb, uint8_t* h_columns, uint8_t* h_siblings);
+/* Queue ZipPlus::prove::<MontyField<4>, _> of a committed request
+ * (phase_prove.rs:85-304) instead of an opening: the server starts the
+ * transcript from the request's root (PcsProverTranscript::
+ * new_from_commitment), draws the field configuration from it
+ * (get_random_field_cfg) and proves at the i128 point (num_vars x 2 words,
+ * copied) with CombR = Int<comb_limbs> and num_openings (<= 256) column
+ * openings, as zp_prove.  The event (kind 2) gives the proof's length; the
+ * request's device memory is released after the proof. */
+int zp_server_prove(zp_server* server, uint64_t request, const int64_t* h_point,
+                    uint32_t num_vars, uint32_t comb_limbs, uint32_t num_openings);
+/* The proof bytes, the field modulus and the canonical evaluation of a
+ * finished proof; *len receives the proof length (ZP_ERR_SHAPE, with *len
+ * set, when cap is smaller).  Ends the request. */
+int zp_server_take_proof(zp_server* server, uint64_t request, uint8_t* h_proof, uint64_t cap,
+                         uint64_t* len, uint8_t h_modulus[32], uint8_t h_eval[32]);
 /* Next finished job; timeout_us < 0 waits without limit.  ZP_ERR_TIMEOUT
  * when none finished in time. */
 int zp_server_poll(zp_server* server, int64_t timeout_us, zp_server_event* event);

@@ -132,3 +132,44 @@ def test_server_device_inputs_and_errors(zg, dev, oracle):
     assert srv.poll(0.01) is None
     assert srv.stats()["hbm_in_use"] == 0
     srv.close()
+
+
+def _point(seed, nv):
+    w = synth.i128_words(seed, nv)
+    return [int(np.uint64(w[i, 0].view(np.uint64))) | (int(w[i, 1]) << 64) for i in range(nv)]
+
+
+def test_server_proofs_equal_direct_prove(zg, dev):
+    # ZipPlus::prove run by the service (transcript from the root, field
+    # drawn, proof written on a worker stream) equals the direct path
+    # (zp_commit + PcsProverTranscript + zp_prove) byte for byte, for every
+    # request of a concurrent mix.
+    reqs = requests(zg, 23, 10, 12, 16)
+    srv = zg.CommitServer(slots=4)
+    ids = {srv.commit(q["pp"], q["types"], q["x"]): i for i, q in enumerate(reqs)}
+    got, pending = {}, 2 * len(reqs)
+    while pending:
+        ev = srv.poll(120)
+        assert ev is not None, "server stalled"
+        assert ev["status"] == 0, ev
+        i = ids[ev["request"]]
+        if ev["kind"] == "commit":
+            q = reqs[i]
+            srv.prove(ev["request"], _point(100 + i, q["n"].bit_length() - 1))
+        else:
+            assert ev["kind"] == "prove"
+            got[i] = ev["proof"]
+        pending -= 1
+    st = srv.stats()
+    assert st["hbm_in_use"] == 0 and st["pending"] == 0
+    srv.close()
+    for i, q in enumerate(reqs):
+        say("direct prove", i, q["kind"], q["n"], q["D"])
+        zp = zg.ZipPlus(q["types"])
+        x = torch.from_numpy(q["x"]).to(dev)
+        hint, comm = zp.commit(q["pp"], [x])
+        t = zg.PcsProverTranscript.new_from_commitment(comm)
+        p = t.get_random_field_cfg()
+        e = zp.prove(t, q["pp"], [x], _point(100 + i, q["n"].bit_length() - 1), hint, p)
+        assert got[i] == (p, e, t.stream())
+        hint.free()

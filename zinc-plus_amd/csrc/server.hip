@@ -13,6 +13,7 @@
 // their device work overlaps.  Every job ends with a synchronisation of its
 // slot's stream, after which its result and timestamps are posted as an
 // event.
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
@@ -56,6 +57,12 @@ struct zp_server {
     uint64_t* h_comb = nullptr;
     uint8_t* h_columns = nullptr;
     uint8_t* h_siblings = nullptr;
+    // Proof (ZipPlus::prove) instead of an opening.
+    bool prove = false;
+    std::vector<int64_t> point;
+    uint32_t num_openings = 0;
+    uint8_t root[32] = {}, modulus[32] = {}, eval[32] = {};
+    std::vector<uint8_t> proof;
   };
   int device = 0;
   uint64_t budget = 0, used = 0, peak = 0;
@@ -95,6 +102,19 @@ uint64_t open_scratch(const zp_commit_desc& d, uint32_t n_cols) {
                                   d.num_rows * 16
                             : 0;
   return comb + (uint64_t)n_cols * (d.num_rows * d.batch * elem + 64 * 32 + 8) + (8u << 20);
+}
+
+// Device scratch of zp_prove with up to n_open openings and CombR limbs.
+uint64_t prove_scratch(const zp_commit_desc& d, uint32_t n_open, uint32_t limbs) {
+  const zp_code* c = d.code;
+  const uint64_t elem = cw_elem_bytes_of(c, d.eval_type, d.degree_plus_one);
+  const uint32_t Dk = d.eval_type == ZP_EVAL_BINPOLY ? d.degree_plus_one : 32;
+  const uint64_t rows = d.num_rows * d.batch;
+  uint64_t b = rows * Dk * 32 + (c->row_len + 2 * d.num_rows) * 32 + (uint64_t)d.batch * Dk * 48 +
+               d.num_rows * 16 + combined_row_scratch_bytes(Dk, c->row_len, d.batch, d.num_rows) +
+               c->row_len * limbs * 8 + (uint64_t)n_open * (rows * elem + 24 + 64 * 32 + 8);
+  if (d.eval_type != ZP_EVAL_BINPOLY) b += rows * c->row_len * 8;
+  return b + (8u << 20);
 }
 
 uint64_t resident_bytes(const zp_commit_desc& d) {
@@ -204,6 +224,27 @@ int run_open(zp_server::Request& r, hipStream_t s) {
   return rc;
 }
 
+// ZipPlus::prove of a committed request on the worker's stream: transcript
+// from the root (PcsProverTranscript::new_from_commitment), field config
+// drawn from it, then zp_prove; the proof bytes stay with the request.
+int run_prove(zp_server::Request& r, hipStream_t s) {
+  const zp_commit_desc& d = r.desc;
+  zp_transcript* t = nullptr;
+  ZP_TRY(zp_transcript_create(&t));
+  int rc = zp_transcript_absorb_slice(t, r.root, 32);
+  if (rc == ZP_OK) rc = zp_transcript_random_field_cfg(t, r.modulus);
+  if (rc == ZP_OK)
+    rc = zp_prove(d.code, d.eval_type, d.degree_plus_one, r.comb_limbs, r.d_evals, d.num_rows,
+                  d.batch, r.hint, r.point.data(), (uint32_t)(r.point.size() / 2), r.modulus,
+                  r.num_openings, t, r.eval, s);
+  if (rc == ZP_OK) {
+    r.proof.resize(zp_transcript_stream_len(t));
+    rc = zp_transcript_stream(t, r.proof.data(), r.proof.size());
+  }
+  zp_transcript_destroy(t);
+  return rc;
+}
+
 void worker(zp_server* sv, uint32_t slot) {
   (void)hipSetDevice(sv->device);
   hipStream_t s = sv->streams[slot];
@@ -243,13 +284,19 @@ void worker(zp_server* sv, uint32_t slot) {
     }
     zp_server_event ev{};
     ev.request = id;
-    ev.kind = is_open ? 1 : 0;
+    ev.kind = is_open ? (r->prove ? 2 : 1) : 0;
     const Clock::time_point t_arrive = is_open ? r->t_open : r->t_commit;
     const Clock::time_point t_start = Clock::now();
     int rc = ZP_OK;
     if (!is_open) {
       rc = pin ? run_commit(sv, *r, s, pin) : fail(ZP_ERR_OOM, "pinned staging failed");
-      if (rc == ZP_OK) std::memcpy(ev.root, pin, 32);
+      if (rc == ZP_OK) {
+        std::memcpy(ev.root, pin, 32);
+        std::memcpy(r->root, pin, 32);
+      }
+    } else if (r->prove) {
+      rc = run_prove(*r, s);
+      ev.proof_bytes = r->proof.size();
     } else {
       rc = run_open(*r, s);
     }
@@ -263,7 +310,12 @@ void worker(zp_server* sv, uint32_t slot) {
       std::strncpy(ev.message, msg, sizeof(ev.message) - 1);
     }
     // A failed commit or any opening ends the request: its memory returns.
-    if (is_open || rc != ZP_OK) {
+    // A finished proof keeps only its bytes until zp_server_take_proof.
+    if (is_open && r->prove && rc == ZP_OK) {
+      release(sv, *r);
+      std::lock_guard<std::mutex> lk(sv->mu);
+      r->state = 5;
+    } else if (is_open || rc != ZP_OK) {
       release(sv, *r);
       std::lock_guard<std::mutex> lk(sv->mu);
       sv->req.erase(id);
@@ -326,7 +378,8 @@ int zp_server_commit(zp_server* sv, const zp_commit_desc* desc, const void* h_ev
   r.h_evals = h_evals;
   r.d_evals = d_evals;
   r.footprint = resident_bytes(*desc) +
-                std::max(commit_scratch(*desc), open_scratch(*desc, 256));
+                std::max({commit_scratch(*desc), open_scratch(*desc, 256),
+                          prove_scratch(*desc, 256, 8)});
   r.t_commit = Clock::now();
   std::lock_guard<std::mutex> lk(sv->mu);
   const uint64_t id = sv->next_id++;
@@ -388,6 +441,47 @@ int zp_server_open(zp_server* sv, uint64_t request, uint32_t comb_limbs, const i
   r.state = 3;
   sv->open_q.push_back(request);
   sv->cv_work.notify_one();
+  return ZP_OK;
+}
+
+int zp_server_prove(zp_server* sv, uint64_t request, const int64_t* h_point, uint32_t num_vars,
+                    uint32_t comb_limbs, uint32_t num_openings) {
+  if (!sv) return fail(ZP_ERR_INVALID_PARAM, "null server");
+  if (num_vars && !h_point) return fail(ZP_ERR_INVALID_PARAM, "null point");
+  if (num_openings > 256) return fail(ZP_ERR_INVALID_PARAM, "at most 256 column openings");
+  if (comb_limbs < 3 || comb_limbs > 8)
+    return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->req.find(request);
+  if (it == sv->req.end()) return fail(ZP_ERR_INVALID_PARAM, "unknown request");
+  zp_server::Request& r = it->second;
+  if (r.state != 2) return fail(ZP_ERR_INVALID_PARAM, "request is not committed yet");
+  r.prove = true;
+  r.point.assign(h_point, h_point + 2ull * num_vars);
+  r.comb_limbs = comb_limbs;
+  r.num_openings = num_openings;
+  r.t_open = Clock::now();
+  r.state = 3;
+  sv->open_q.push_back(request);
+  sv->cv_work.notify_one();
+  return ZP_OK;
+}
+
+int zp_server_take_proof(zp_server* sv, uint64_t request, uint8_t* h_proof, uint64_t cap,
+                         uint64_t* len, uint8_t h_modulus[32], uint8_t h_eval[32]) {
+  if (!sv || !len) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  auto it = sv->req.find(request);
+  if (it == sv->req.end()) return fail(ZP_ERR_INVALID_PARAM, "unknown request");
+  zp_server::Request& r = it->second;
+  if (r.state != 5) return fail(ZP_ERR_INVALID_PARAM, "no finished proof for this request");
+  *len = r.proof.size();
+  if (cap < r.proof.size()) return fail(ZP_ERR_SHAPE, "buffer smaller than the proof");
+  if (!h_proof && !r.proof.empty()) return fail(ZP_ERR_INVALID_PARAM, "null buffer");
+  if (!r.proof.empty()) std::memcpy(h_proof, r.proof.data(), r.proof.size());
+  if (h_modulus) std::memcpy(h_modulus, r.modulus, 32);
+  if (h_eval) std::memcpy(h_eval, r.eval, 32);
+  sv->req.erase(it);
   return ZP_OK;
 }
 

@@ -689,6 +689,24 @@ class CommitServer:
         self._opening[request] = (limbs, cols, comb, columns, sib, al, co,
                                   sizes[1].value, sizes[2].value)
 
+    def prove(self, request, point, comb_limbs=None, num_openings=None):
+        """Queue ZipPlus::prove of a committed request at an i128 point
+        (the server draws the transcript from the root); the event's
+        "proof" is (modulus, eval, proof bytes)."""
+        if request not in self._keep:
+            raise InvalidPcsParam(L.ZP_ERR_INVALID_PARAM, "unknown request")
+        _, types, _, _ = self._keep[request]
+        pt = np.zeros((max(1, len(point)), 2), dtype=np.int64)
+        for i, v in enumerate(point):
+            u = int(v) & ((1 << 128) - 1)
+            pt[i, 0] = np.uint64(u & ((1 << 64) - 1)).view(np.int64)
+            pt[i, 1] = np.uint64(u >> 64).view(np.int64)
+        _check(self._lib.zp_server_prove(
+            self._h, request, pt.ctypes.data_as(ctypes.c_void_p), len(point),
+            comb_limbs or types.comb_limbs,
+            types.num_column_openings if num_openings is None else num_openings))
+        self._opening[request] = "prove"
+
     def poll(self, timeout=None):
         """Next finished job as a dict, or None after `timeout` seconds."""
         ev = L.ServerEvent()
@@ -697,7 +715,7 @@ class CommitServer:
         if rc == L.ZP_ERR_TIMEOUT:
             return None
         _check(rc)
-        out = {"request": ev.request, "kind": "commit" if ev.kind == 0 else "open",
+        out = {"request": ev.request, "kind": ("commit", "open", "prove")[ev.kind],
                "status": ev.status, "queued_us": ev.queued_us, "service_us": ev.service_us,
                "latency_us": ev.latency_us,
                "message": ev.message.decode(errors="replace")}
@@ -705,6 +723,18 @@ class CommitServer:
             out["root"] = bytes(ev.root)
             if ev.status != L.ZP_OK:
                 self._keep.pop(ev.request, None)
+        elif ev.kind == 2:
+            self._keep.pop(ev.request, None)
+            self._opening.pop(ev.request, None)
+            if ev.status == L.ZP_OK:
+                buf = np.zeros(max(1, ev.proof_bytes), dtype=np.uint8)
+                n = ctypes.c_uint64()
+                mod, evl = (ctypes.c_uint8 * 32)(), (ctypes.c_uint8 * 32)()
+                _check(self._lib.zp_server_take_proof(
+                    self._h, ev.request, buf.ctypes.data_as(ctypes.c_void_p), ev.proof_bytes,
+                    ctypes.byref(n), mod, evl))
+                out["proof"] = (int.from_bytes(bytes(mod), "little"),
+                                int.from_bytes(bytes(evl), "little"), buf[:n.value].tobytes())
         else:
             pp, types, batch, _ = self._keep.pop(ev.request)
             limbs, cols, comb, columns, sib, _, _, cb, sb = self._opening.pop(ev.request)

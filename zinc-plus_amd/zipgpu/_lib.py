@@ -55,7 +55,8 @@ class ServerEvent(ctypes.Structure):
     _fields_ = [("request", ctypes.c_uint64), ("kind", ctypes.c_int32),
                 ("status", ctypes.c_int32), ("root", ctypes.c_uint8 * 32),
                 ("queued_us", ctypes.c_double), ("service_us", ctypes.c_double),
-                ("latency_us", ctypes.c_double), ("message", ctypes.c_char * 128)]
+                ("latency_us", ctypes.c_double), ("message", ctypes.c_char * 128),
+                ("proof_bytes", ctypes.c_uint64)]
 
 
 # name -> (restype, argtypes); mirrors include/zipgpu.h one for one.
@@ -100,6 +101,9 @@ SIGNATURES = {
                                     ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
     "zp_server_open": (_int, [_c_void_p, _u64, _u32, _c_void_p, _c_void_p, _c_void_p, _u32,
                               _c_void_p, _c_void_p, _c_void_p]),
+    "zp_server_prove": (_int, [_c_void_p, _u64, _c_void_p, _u32, _u32, _u32]),
+    "zp_server_take_proof": (_int, [_c_void_p, _u64, _c_void_p, _u64, ctypes.POINTER(_u64),
+                                    _c_void_p, _c_void_p]),
     "zp_server_poll": (_int, [_c_void_p, ctypes.c_int64, _c_void_p]),
     "zp_server_stats": (_int, [_c_void_p] + [ctypes.POINTER(_u64)] * 5),
     "zp_server_destroy": (None, [_c_void_p]),
