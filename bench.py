@@ -273,14 +273,17 @@ def served_requests(zipgpu, synth, seed, count, lo, hi):
                     "x": synth.binary_polys(r["seed"], r["n"], r["D"]),
                     "al": synth.i128_words(r["seed"] + 1, r["D"]),
                     "co": synth.i128_words(r["seed"] + 2, pp.num_rows),
-                    "cols": (synth.splitmix64(r["seed"] + 3, 147) % np.uint64(N))})
+                    "cols": (synth.splitmix64(r["seed"] + 3, 147) % np.uint64(N)),
+                    "point": [int(v) for v in synth.splitmix64(r["seed"] + 4,
+                                                               r["n"].bit_length() - 1)]})
     return out, codes
 
 
-def serve(zipgpu, reqs, slots, arrivals, open_delays):
+def serve(zipgpu, reqs, slots, arrivals, open_delays, prove=False):
     """Run the requests through a CommitServer: commit i arrives at
-    arrivals[i] (seconds from the start), its opening open_delays[i] after
-    its root is back.  -> (wall seconds, commit latencies us, open latencies us)."""
+    arrivals[i] (seconds from the start), its opening (or, with prove, its
+    full ZipPlus::prove at the point q["point"]) open_delays[i] after its
+    root is back.  -> (wall seconds, commit latencies us, open latencies us)."""
     import heapq
     srv = zipgpu.CommitServer(slots=slots)
     t0 = time.perf_counter()
@@ -294,6 +297,8 @@ def serve(zipgpu, reqs, slots, arrivals, open_delays):
             q = reqs[i]
             if kind == 0:
                 ids[srv.commit(q["pp"], q["types"], q["x"])] = i
+            elif prove:
+                srv.prove(q["rid"], q["point"])
             else:
                 srv.open(q["rid"], q["al"], q["co"], q["cols"])
         wait = max(0.0, due[0][0] - (time.perf_counter() - t0)) if due else 1.0
@@ -340,14 +345,20 @@ def leg_served(torch, zipgpu, synth, dev):
         s1.append(serve(zipgpu, reqs, 1, [0.0] * count, [0.0] * count)[0])
     sat4, sat1 = float(np.median(s4)), float(np.median(s1))
     same = roots == [q["root"] for q in reqs]
+    # The same arrivals with full proofs (Fiat-Shamir transcript from each
+    # root, b, combined row, 147 openings, proof bytes) instead of openings.
+    pwall, _, lp, _ = serve(zipgpu, reqs, 4, arrivals, delays, prove=True)
     pct = lambda a, p: float(np.percentile(np.asarray(a), p))  # noqa: E731
     return {"workload": f"{count} commit+open requests, Poisson {rate:.0f}/s, log2 n ~ U{{14..22}}, "
                         "D 32/64, IPRS/RAA, host inputs; openings: combined row Int<5> + 147 "
-                        "columns with Merkle paths, exp(1 ms) after each root; 4 slots",
+                        "columns with Merkle paths, exp(1 ms) after each root; 4 slots; the same arrivals "
+                        "again with full proofs (prove_*: transcript from the root, 147 openings)",
             "total_evals": total, "wall_s": wall, "evals_per_s": total / wall,
             "commit_latency_us_p50": pct(lc, 50), "commit_latency_us_p99": pct(lc, 99),
             "open_latency_us_p50": pct(lo, 50), "open_latency_us_p99": pct(lo, 99),
             "hbm_peak_bytes": peak,
+            "prove_wall_s": pwall, "prove_latency_us_p50": pct(lp, 50),
+            "prove_latency_us_p99": pct(lp, 99),
             "saturated_4_slots_evals_per_s": total / sat4,
             "one_at_a_time_evals_per_s": total / sat1,
             "served_vs_one_at_a_time": sat1 / sat4,
