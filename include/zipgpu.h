@@ -48,9 +48,13 @@ typedef enum {
 typedef enum { ZP_CODE_IPRS = 0, ZP_CODE_RAA = 1, ZP_CODE_RAA_SIGNFLIP = 2 } zp_code_kind;
 
 typedef enum {
-  ZP_EVAL_I32 = 0,    /* Eval = i32, Cw = i128 (zip-plus/benches/zip_benches.rs:25-40) */
-  ZP_EVAL_BINPOLY = 1 /* Eval = BinaryPoly<D>, Cw = [i64;D] (IPRS) or [i32;D] (RAA),
-                         zip-plus/benches/zip_plus_benches.rs:37-78 */
+  ZP_EVAL_I32 = 0,     /* Eval = i32, Cw = i128 (zip-plus/benches/zip_benches.rs:25-40) */
+  ZP_EVAL_BINPOLY = 1, /* Eval = BinaryPoly<D>, Cw = [i64;D] (IPRS) or [i32;D] (RAA),
+                          zip-plus/benches/zip_plus_benches.rs:37-78 */
+  ZP_EVAL_I64 = 2,     /* Eval = i64, Cw = i128 (IntZipTypes, protocol/src/lib.rs:546-559) */
+  ZP_EVAL_DENSE_I64 = 3 /* Eval = DensePolynomial<i64, D>, Cw = [i64;D] (IPRS) or
+                           [Int<4>;D] (RAA) (ArbitraryPolyZipTypesIprs/Raa,
+                           protocol/src/lib.rs:490-541) */
 } zp_eval_type;
 
 typedef struct zp_code zp_code;

@@ -548,3 +548,43 @@ def verify(ts, p, root, point, ev, encode_wide, num_rows, row_len, N, elem, batc
     if ts.pos != len(ts.stream):
         raise ValueError("trailing proof bytes")
     return True
+
+
+# ---------------------------------------------------------------------------
+# i64 and DensePolynomial<i64, D> evaluations (protocol/src/lib.rs:490-559):
+# the code applied to each coefficient plane in the codeword type
+# (Int<L> arithmetic mod 2^(64L)): IPRS i64 -> i128 (L = 2), IPRS
+# DensePolynomial<i64, D> -> [i64; D] (L = 1), RAA i64 -> i128 (L = 2), RAA
+# DensePolynomial<i64, D> -> [Int<4>; D] (L = 4).
+# ---------------------------------------------------------------------------
+
+def i64_plane_limbs(kind, eval_type):
+    if eval_type == "i64":
+        return 2
+    return 1 if kind == "iprs" else 4
+
+
+def encode_i64_planes(kind, evals, num_rows, row_len, W, L, depth=-1, rep=4):
+    """evals: int64 [num_rows, row_len, W] -> uint64 [num_rows, N, W, L]."""
+    x = np.ascontiguousarray(evals, dtype=np.int64).reshape(num_rows, row_len, W)
+    N = row_len * rep
+    out = np.zeros((num_rows, N, W, L), dtype=np.uint64)
+    for c in range(W):
+        limbs = sign_extend_limbs(x[:, :, c:c + 1], L)
+        if kind == "iprs":
+            out[:, :, c, :] = iprs_encode_wide(limbs, row_len, depth, L, rep)
+        else:
+            out[:, :, c, :] = raa_encode_wide(limbs, row_len, L, rep,
+                                              sign_flip=kind == "raa_signflip")
+    return out
+
+
+def commit_i64(kind, evals, num_rows, row_len, batch=1, D=1, eval_type="i64", depth=-1, rep=4):
+    """ZipPlus::commit of i64 / DensePolynomial<i64, D> polys -> dict(root, cw, layers)."""
+    W = 1 if eval_type == "i64" else D
+    L = i64_plane_limbs(kind, eval_type)
+    cw = encode_i64_planes(kind, evals, batch * num_rows, row_len, W, L, depth, rep)
+    N = row_len * rep
+    elem = W * L * 8
+    layers = merkle_layers(merkle_leaves(cw.view(np.uint8).reshape(-1), batch * num_rows, N, elem))
+    return {"root": layers[-1].tobytes(), "cw": cw, "layers": layers, "elem": elem}

@@ -230,3 +230,44 @@ def test_verify_rejects_planted_mistakes(oracle, where):
     with pytest.raises(ValueError):
         oracle.verify(tv, p, c["root"], point, e + (where == "eval"), _encoder(oracle, kind,
                       row_len, L), num_rows, row_len, N, elem, D=D, L=L, num_openings=5)
+
+
+# ---------------------------------------------------------------------------
+# i64 and DensePolynomial<i64, D> evaluations (the protocol's commits).
+# ---------------------------------------------------------------------------
+
+def test_iprs_i64_small_equals_i32_path(oracle):
+    # |x| < 2^31: Eval = i64 -> Cw = i128 is the pinned i32 -> i128 encode.
+    row_len, rows = 512, 3
+    x = synth.i32_evals(8, rows * row_len)
+    got = oracle.encode_i64_planes("iprs", x.astype(np.int64), rows, row_len, 1, 2)
+    want = oracle.iprs_encode_i32(x, rows, row_len).view(np.uint64).reshape(rows, -1, 1, 2)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("kind", ["iprs", "raa", "raa_signflip"])
+def test_dense_binary_coefficients_equal_binary_path(oracle, kind):
+    # DensePolynomial<i64, D> with 0/1 coefficients encodes like BinaryPoly<D>
+    # (IPRS: both [i64; D]; RAA: [Int<4>; D] against [i32; D] sign-extended).
+    row_len, rows, D = 256, 2, 8
+    bits = synth.binary_polys(9, rows * row_len, D).view(np.uint64)
+    dense = ((bits[:, None] >> np.arange(D, dtype=np.uint64)) & np.uint64(1)).astype(np.int64)
+    L = 1 if kind == "iprs" else 4
+    got = oracle.encode_i64_planes(kind, dense, rows, row_len, D, L)
+    if kind == "iprs":
+        want = oracle.iprs_encode_binary(bits, rows, row_len, -1, D).view(np.uint64)
+        assert np.array_equal(got[..., 0], want.reshape(rows, -1, D))
+    else:
+        want = oracle.raa_encode_binary(bits, rows, row_len, D,
+                                        sign_flip=kind == "raa_signflip").astype(np.int64)
+        assert np.array_equal(got, oracle.sign_extend_limbs(want.reshape(rows, -1, D, 1), 4))
+
+
+def test_raa_i64_small_equals_i64_path(oracle):
+    row_len, rows = 128, 2
+    x = synth.i32_evals(10, rows * row_len).astype(np.int64)
+    got = oracle.encode_i64_planes("raa", x, rows, row_len, 1, 2)
+    for r in range(rows):
+        want = oracle.raa_encode_i64(x[r * row_len:(r + 1) * row_len])
+        assert np.array_equal(got[r, :, 0, :], oracle.sign_extend_limbs(
+            np.asarray(want, dtype=np.int64).reshape(-1, 1), 2))

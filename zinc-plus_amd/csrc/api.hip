@@ -26,16 +26,26 @@ int g_force_iprs = -1, g_force_raa = -1, g_force_overlap = -1;
 
 
 uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D) {
+  if (eval_type == ZP_EVAL_I64) return 16;  // i128 under either code
   if (c->kind == ZP_CODE_IPRS) {
     if (eval_type == ZP_EVAL_I32) return 16;
     if (eval_type == ZP_EVAL_BINPOLY) return 8ull * D;
+    if (eval_type == ZP_EVAL_DENSE_I64) return 8ull * D;
   } else if (eval_type == ZP_EVAL_BINPOLY) {
     return 4ull * D;
+  } else if (eval_type == ZP_EVAL_DENSE_I64) {
+    return 32ull * D;  // Int<4> coefficients
   }
   return 0;
 }
 
-uint64_t eval_bytes_of(int eval_type) { return eval_type == ZP_EVAL_I32 ? 4 : 8; }
+uint64_t eval_bytes_of(int eval_type, uint32_t D) {
+  switch (eval_type) {
+    case ZP_EVAL_I32: return 4;
+    case ZP_EVAL_DENSE_I64: return 8ull * D;
+    default: return 8;
+  }
+}
 
 namespace {
 
@@ -97,9 +107,11 @@ int pick(int forced, int tuned, int dflt) { return forced >= 0 ? forced : tuned 
 int check_types(const zp_code* c, int eval_type, uint32_t D) {
   if (eval_type == ZP_EVAL_BINPOLY && (D == 0 || D > 64))
     return fail(ZP_ERR_INVALID_PARAM, "BinaryPoly degree bound must be within 1..64");
+  if (eval_type == ZP_EVAL_DENSE_I64 && (D == 0 || D > 64))
+    return fail(ZP_ERR_INVALID_PARAM, "DensePolynomial degree bound must be within 1..64");
   if (cw_elem_bytes_of(c, eval_type, D) == 0)
     return fail(ZP_ERR_INVALID_PARAM, "unsupported eval type for this code");
-  if (c->kind != ZP_CODE_IPRS) {
+  if (c->kind != ZP_CODE_IPRS && eval_type == ZP_EVAL_BINPOLY) {
     // raa.rs:52-69: 1 + 2*log2(row_len) + 2*log2(REP) bits must fit i32.
     const uint64_t w = 1 + 2 * (uint64_t)__builtin_ctzll(c->row_len) +
                        2 * (uint64_t)__builtin_ctzll(c->rep);
@@ -113,6 +125,13 @@ int check_types(const zp_code* c, int eval_type, uint32_t D) {
 // Encodes with the variant in force for this code (types already checked).
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
                         uint64_t num_rows, void* out, hipStream_t s) {
+  // i64 and DensePolynomial<i64, D> evals: the Int<L> encoders per
+  // coefficient plane (wrapping in the Cw type, as the unchecked build).
+  if (eval_type == ZP_EVAL_I64)
+    return launch_encode_i64_planes(c, 1, 2, (const int64_t*)evals, num_rows, out, s);
+  if (eval_type == ZP_EVAL_DENSE_I64)
+    return launch_encode_i64_planes(c, D, c->kind == ZP_CODE_IPRS ? 1 : 4,
+                                    (const int64_t*)evals, num_rows, out, s);
   if (c->kind == ZP_CODE_IPRS) {
     if (eval_type == ZP_EVAL_BINPOLY) {
       if (iprs_mfma_eligible(c, D) && pick(g_force_iprs, c->tune_iprs, 0) == 0)
@@ -135,7 +154,7 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
                            uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
                            bool allow_overlap) {
   const uint64_t N = h->N, elem = h->elem, rows_all = num_rows * batch;
-  const uint64_t in_stride = num_rows * code->row_len * eval_bytes_of(eval_type);
+  const uint64_t in_stride = num_rows * code->row_len * eval_bytes_of(eval_type, D);
   const uint64_t rpc = (elem <= 1024 && 1024 % elem == 0) ? 1024 / elem : 0;  // rows per chunk
   const uint64_t nchunks = leaf_chunks(rows_all, (uint32_t)elem);
   const uint32_t kGroups = 4;
@@ -161,7 +180,7 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
   for (uint32_t q = 0; rc == ZP_OK && q < kGroups; q++) {
     rc = encode_rows_variant(
         code, eval_type, D,
-        (const uint8_t*)d_evals + q * rows_q * code->row_len * eval_bytes_of(eval_type), rows_q,
+        (const uint8_t*)d_evals + q * rows_q * code->row_len * eval_bytes_of(eval_type, D), rows_q,
         h->d_cw + q * rows_q * N * elem, s);
     if (rc == ZP_OK && (hipEventRecord(ev[q], s) != hipSuccess ||
                         hipStreamWaitEvent(s2, ev[q], 0) != hipSuccess))

@@ -151,7 +151,7 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
   if (!code) return fail(ZP_ERR_INVALID_PARAM, "null code");
   zp_hint* h = nullptr;
   ZP_TRY(hint_alloc(code, eval_type, D, num_rows, batch, s, &h));
-  const uint64_t words = num_rows * batch * code->row_len * eval_bytes_of(eval_type) / 8;
+  const uint64_t words = num_rows * batch * code->row_len * eval_bytes_of(eval_type, D) / 8;
   uint64_t* evals = nullptr;
   int rc = dev_alloc((void**)&evals, words * 8 + 8, s);
   if (rc == ZP_OK) {
@@ -183,7 +183,9 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
     }
     *field = best_v;
   };
-  if (code->kind == ZP_CODE_IPRS) {
+  if (eval_type == ZP_EVAL_I64 || eval_type == ZP_EVAL_DENSE_I64) {
+    // One encoder (the Int<L> kernels): nothing to tune.
+  } else if (code->kind == ZP_CODE_IPRS) {
     std::vector<int> c = {1};
     if (eval_type == ZP_EVAL_BINPOLY && iprs_mfma_eligible(code, D)) c = {0, 1};
     tune(&code->tune_iprs, c, encode_all);

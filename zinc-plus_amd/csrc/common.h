@@ -143,6 +143,8 @@ int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void*
 // wide.hip: LinearCode::encode_wide over Int<L>, L in 1..8.
 int launch_encode_wide(const zp_code* c, uint32_t L, const uint64_t* in, uint64_t rows,
                        uint64_t* out, hipStream_t s);
+int launch_encode_i64_planes(const zp_code* c, uint32_t D, uint32_t L, const int64_t* in,
+                             uint64_t rows, void* out, hipStream_t s);
 // prove.hip: column values and Merkle paths of n_cols columns of a hint
 // (columns: n_cols x rows x elem bytes; paths: n_cols x (height-1) x 32).
 int launch_gather_openings(const zp_hint* h, const uint64_t* d_cols, uint32_t n_cols,
@@ -150,7 +152,7 @@ int launch_gather_openings(const zp_hint* h, const uint64_t* d_cols, uint32_t n_
 inline bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
 int check_types(const zp_code* c, int eval_type, uint32_t D);
 uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D);
-uint64_t eval_bytes_of(int eval_type);
+uint64_t eval_bytes_of(int eval_type, uint32_t D);
 int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
                      uint8_t* leaves, hipStream_t s);
 uint64_t leaf_chunks(uint64_t nrows, uint32_t elem);

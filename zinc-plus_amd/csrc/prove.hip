@@ -257,6 +257,8 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   if (!code || !hint || !t || !h_modulus || !h_eval || (num_vars && !h_point) || !d_evals)
     return fail(ZP_ERR_INVALID_PARAM, "null argument");
   ZP_TRY(check_types(code, eval_type, D));
+  if (eval_type != ZP_EVAL_BINPOLY && eval_type != ZP_EVAL_I32)
+    return fail(ZP_ERR_INVALID_PARAM, "prove supports BinaryPoly and i32 evals");
   if (L < 3 || L > 8) return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
   const uint64_t row_len = code->row_len, N = code->N;
   if (batch == 0) return fail(ZP_ERR_SHAPE, "Batch must contain at least one polynomial");

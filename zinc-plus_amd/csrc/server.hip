@@ -120,7 +120,7 @@ uint64_t prove_scratch(const zp_commit_desc& d, uint32_t n_open, uint32_t limbs)
 uint64_t resident_bytes(const zp_commit_desc& d) {
   const zp_code* c = d.code;
   const uint64_t elem = cw_elem_bytes_of(c, d.eval_type, d.degree_plus_one);
-  const uint64_t in = (uint64_t)d.batch * d.num_rows * c->row_len * eval_bytes_of(d.eval_type);
+  const uint64_t in = (uint64_t)d.batch * d.num_rows * c->row_len * eval_bytes_of(d.eval_type, d.degree_plus_one);
   return d.batch * d.num_rows * c->N * elem + (2 * c->N - 1) * 32 + in;
 }
 
@@ -149,7 +149,7 @@ int run_commit(zp_server* sv, zp_server::Request& r, hipStream_t s, uint8_t* pin
   r.home = s;
   if (r.h_evals) {
     const uint64_t in = (uint64_t)d.batch * d.num_rows * d.code->row_len *
-                        eval_bytes_of(d.eval_type);
+                        eval_bytes_of(d.eval_type, d.degree_plus_one);
     void* p = nullptr;
     ZP_TRY(dev_alloc(&p, in, s));
     r.d_evals = p;

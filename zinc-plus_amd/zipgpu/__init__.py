@@ -33,7 +33,7 @@ from . import _lib as L
 
 __all__ = [
     "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory",
-    "ZipTypes", "BinPolyTypes", "IntTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
+    "ZipTypes", "BinPolyTypes", "IntTypes", "I64Types", "DensePolyTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
     "merkle_root", "lib", "set_option", "BatchCommit", "CommitServer", "PcsProverTranscript",
 ]
@@ -104,25 +104,35 @@ class ZipTypes:
     """The Eval / Cw / CombR choice of a ZipTypes impl (pcs/structs.rs:15-60).
 
     ``eval_type`` is "binpoly" (Eval = BinaryPoly<D>, u64 bit-packed, Cw =
-    [i64;D] under IPRS and [i32;D] under RAA) or "i32" (Eval = i32, Cw =
-    i128).  ``comb_limbs`` is L of CombR = Int<L>.
+    [i64;D] under IPRS and [i32;D] under RAA), "i32" (Eval = i32, Cw =
+    i128), "i64" (Eval = i64, Cw = i128) or "dense_i64" (Eval =
+    DensePolynomial<i64, D>, Cw = [i64;D] under IPRS and [Int<4>;D] under
+    RAA).  ``comb_limbs`` is L of CombR = Int<L>.
     """
 
+    _CODES = {"binpoly": L.ZP_EVAL_BINPOLY, "i32": L.ZP_EVAL_I32, "i64": L.ZP_EVAL_I64,
+              "dense_i64": L.ZP_EVAL_DENSE_I64}
+
     def __init__(self, eval_type, degree_plus_one=1, comb_limbs=5, num_column_openings=147):
-        if eval_type not in ("binpoly", "i32"):
+        if eval_type not in self._CODES:
             raise ValueError(eval_type)
         self.eval_type = eval_type
-        self.degree_plus_one = degree_plus_one if eval_type == "binpoly" else 1
+        self.degree_plus_one = degree_plus_one if eval_type in ("binpoly", "dense_i64") else 1
         self.comb_limbs = comb_limbs
         self.num_column_openings = num_column_openings
 
     @property
     def code(self):
-        return L.ZP_EVAL_BINPOLY if self.eval_type == "binpoly" else L.ZP_EVAL_I32
+        return self._CODES[self.eval_type]
 
     @property
     def eval_dtype(self):
-        return torch.int64 if self.eval_type == "binpoly" else torch.int32
+        return torch.int32 if self.eval_type == "i32" else torch.int64
+
+    @property
+    def eval_words(self):
+        """Elements of eval_dtype per evaluation (D coefficients for dense_i64)."""
+        return self.degree_plus_one if self.eval_type == "dense_i64" else 1
 
 
 def BinPolyTypes(degree_plus_one, comb_limbs=5):
@@ -133,6 +143,17 @@ def BinPolyTypes(degree_plus_one, comb_limbs=5):
 def IntTypes(comb_limbs=3):
     """BenchZipTypes (zip-plus/benches/zip_benches.rs:25-40): i32 -> i128."""
     return ZipTypes("i32", 1, comb_limbs)
+
+
+def I64Types(comb_limbs=8):
+    """IntZipTypes of the protocol (protocol/src/lib.rs:546-559): i64 -> i128."""
+    return ZipTypes("i64", 1, comb_limbs)
+
+
+def DensePolyTypes(degree_plus_one, comb_limbs=8):
+    """ArbitraryPolyZipTypesIprs / ...Raa (protocol/src/lib.rs:490-541):
+    DensePolynomial<i64, D> -> [i64; D] (IPRS) or [Int<4>; D] (RAA)."""
+    return ZipTypes("dense_i64", degree_plus_one, comb_limbs)
 
 
 class LinearCode:
@@ -162,9 +183,13 @@ class LinearCode:
 
     def cw_shape(self, types):
         """Trailing shape and dtype of one codeword element."""
+        if types.eval_type in ("i32", "i64"):
+            return (2,), torch.int64  # i128 as two LE words
+        if types.eval_type == "dense_i64":
+            if self.KIND == L.ZP_CODE_IPRS:
+                return (types.degree_plus_one,), torch.int64
+            return (types.degree_plus_one, 4), torch.int64  # Int<4> limbs
         if self.KIND == L.ZP_CODE_IPRS:
-            if types.eval_type == "i32":
-                return (2,), torch.int64  # i128 as two LE words
             return (types.degree_plus_one,), torch.int64
         return (types.degree_plus_one,), torch.int32
 
@@ -175,7 +200,7 @@ class LinearCode:
         """Encode num_rows consecutive rows of a flat device eval tensor."""
         if evals.dtype != types.eval_dtype:
             raise ValueError(f"evals must be {types.eval_dtype}")
-        if evals.numel() != num_rows * self.row_len():
+        if evals.numel() != num_rows * self.row_len() * types.eval_words:
             raise ShapeError(L.ZP_ERR_SHAPE,
                              f"Input length {evals.numel()} does not match "
                              f"{num_rows} rows of {self.row_len()}")
@@ -473,7 +498,7 @@ class ZipPlus:
             t = torch.stack([p.reshape(-1) for p in polys])
         if t.shape[0] == 0:
             raise ShapeError(L.ZP_ERR_SHAPE, "Batch must contain at least one polynomial")
-        expected = pp.num_rows * pp.linear_code.row_len()
+        expected = pp.num_rows * pp.linear_code.row_len() * self.types.eval_words
         if t.shape[1] != expected:
             raise ShapeError(L.ZP_ERR_SHAPE,
                              f"Polynomial has an incorrect number of evaluations ({t.shape[1]}) "
@@ -579,7 +604,7 @@ class BatchCommit:
 
     def _check_input(self, i, t):
         pp, types, batch = self.items[i]
-        n = batch * pp.num_rows * pp.linear_code.row_len()
+        n = batch * pp.num_rows * pp.linear_code.row_len() * types.eval_words
         if t.dtype != types.eval_dtype or t.numel() != n:
             raise ShapeError(L.ZP_ERR_SHAPE, f"batch item {i}: expected {n} evals of "
                                              f"{types.eval_dtype}")
@@ -642,7 +667,7 @@ class CommitServer:
         self.close()
 
     def commit(self, pp, types, evals, batch=1):
-        n = batch * pp.num_rows * pp.linear_code.row_len()
+        n = batch * pp.num_rows * pp.linear_code.row_len() * types.eval_words
         desc = L.CommitDesc(pp.linear_code._h.value, types.code, types.degree_plus_one,
                             pp.num_rows, batch)
         if isinstance(evals, torch.Tensor):
@@ -650,7 +675,7 @@ class CommitServer:
                 raise ShapeError(L.ZP_ERR_SHAPE, f"expected {n} evals of {types.eval_dtype}")
             h_ptr, d_ptr = None, _dev_ptr(evals)
         else:
-            want = np.int64 if types.eval_type == "binpoly" else np.int32
+            want = np.int32 if types.eval_type == "i32" else np.int64
             evals = np.ascontiguousarray(evals)
             if evals.dtype.itemsize != np.dtype(want).itemsize or evals.size != n:
                 raise ShapeError(L.ZP_ERR_SHAPE, f"expected {n} evals of {np.dtype(want)}")

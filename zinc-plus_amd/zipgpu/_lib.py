@@ -24,6 +24,8 @@ ZP_CODE_RAA_SIGNFLIP = 2
 
 ZP_EVAL_I32 = 0
 ZP_EVAL_BINPOLY = 1
+ZP_EVAL_I64 = 2
+ZP_EVAL_DENSE_I64 = 3
 
 ZP_BATCH_GRAPH = 1
 ZP_BATCH_AUTOTUNE = 2
