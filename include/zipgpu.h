@@ -301,6 +301,12 @@ void zp_transcript_destroy(zp_transcript* t);
 /* Transcript::absorb_slice (transcript/src/traits.rs:388-392). */
 int zp_transcript_absorb_slice(zp_transcript* t, const uint8_t* bytes, uint64_t len);
 
+/* absorb_slice of each of `count` consecutive entries of entry_bytes bytes:
+ * absorb_public_columns of the protocol (protocol/src/lib.rs:305-316) for
+ * one column of transcription bytes. */
+int zp_transcript_absorb_entries(zp_transcript* t, const uint8_t* bytes, uint64_t entry_bytes,
+                                 uint64_t count);
+
 /* Transcript::get_challenge::<T> with T::NUM_BYTES = num_bytes: the
  * little-endian bytes of T (lib.rs:48-55). */
 int zp_transcript_challenge(zp_transcript* t, uint8_t* h_out, uint32_t num_bytes);

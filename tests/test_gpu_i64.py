@@ -61,3 +61,58 @@ def test_i64_encode_rows_shapes(zg, dev):
     assert tuple(out.shape) == (2, 256, 4, 4)
     with pytest.raises(zg.ShapeError):
         code.encode_rows(x[:-1], 2, types)
+
+
+def _sizes_commit(oracle, kind, x, n, cols, types, eval_type):
+    if eval_type == "binpoly":
+        return oracle.commit(kind, x.view(np.uint64), 1, n, batch=cols, D=types.degree_plus_one)
+    return oracle.commit_i64(kind, x, 1, n, cols, types.degree_plus_one, eval_type)
+
+
+@pytest.mark.parametrize("empty_arb", [False, True])
+def test_step0_commit_matches_oracle(zg, dev, oracle, empty_arb):
+    # ZincPlusPiop::step0_commit (protocol/src/prover.rs:218-262) with the
+    # protocol's IPRS set-up (one row per poly, row_len = poly size):
+    # binary, arbitrary and int witness traces committed, roots and public
+    # entries absorbed.
+    say("step0", empty_arb)
+    n = 1 << 10
+    D = 32
+    kinds = [("binpoly", zg.BinPolyTypes(D, 8)), ("dense_i64", zg.DensePolyTypes(D)),
+             ("i64", zg.I64Types())]
+    params, wit, pub, host_w, host_p = [], [], [], [], []
+    for g, (et, types) in enumerate(kinds):
+        code = zg.IprsCode.new_with_optimal_depth(n)
+        pp = zg.ZipPlus(types).setup(n, code)
+        params.append((pp, types))
+        W = types.eval_words
+        nw = 0 if (empty_arb and et == "dense_i64") else 2
+        if et == "binpoly":
+            xs = [synth.binary_polys(200 + 10 * g + k, n, D) for k in range(nw + 1)]
+        else:
+            xs = [evals_i64(200 + 10 * g + k, n * W, True) for k in range(nw + 1)]
+        host_w.append(xs[1:])
+        host_p.append(xs[:1])
+        wit.append([torch.from_numpy(v).to(dev) for v in xs[1:]])
+        pub.append([torch.from_numpy(v).to(dev) for v in xs[:1]])
+    hints, comms, t = zg.step0_commit(params, wit, pub)
+    ts = oracle.Transcript()
+    for g, (et, types) in enumerate(kinds):
+        cols = host_w[g]
+        if not cols:
+            assert hints[g] is None and comms[g].root == bytes(32) and comms[g].batch_size == 0
+            ts.absorb_slice(bytes(32))
+            continue
+        want = _sizes_commit(oracle, "iprs", np.concatenate(cols), n, len(cols), types, et)
+        assert comms[g].root == want["root"] and comms[g].batch_size == len(cols)
+        ts.absorb_slice(want["root"])
+    for g, (et, types) in enumerate(kinds):
+        entry = 8 * types.eval_words
+        for col in host_p[g]:
+            raw = col.tobytes()
+            for i in range(len(raw) // entry):
+                ts.absorb_slice(raw[i * entry:(i + 1) * entry])
+    assert t.get_challenge(32) == ts.challenge(32)
+    for h in hints:
+        if h is not None:
+            h.free()

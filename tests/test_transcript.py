@@ -70,3 +70,13 @@ def test_squeeze_zero_cap_is_an_error(zgh):
     t = zgh.PcsProverTranscript()
     with pytest.raises(zgh.ShapeError):
         t.squeeze_challenge_idx(0)
+
+
+def test_absorb_entries_is_absorb_slice_per_entry(zgh, oracle):
+    import numpy as np
+    data = bytes(np.random.default_rng(3).integers(0, 256, 40 * 24, dtype=np.uint8))
+    t, o = zgh.PcsProverTranscript(), oracle.Transcript()
+    t.absorb_entries(data, 40)
+    for i in range(24):
+        o.absorb_slice(data[40 * i:40 * (i + 1)])
+    assert t.get_challenge(32) == o.challenge(32)

@@ -367,6 +367,17 @@ int zp_transcript_absorb_slice(zp_transcript* t, const uint8_t* bytes, uint64_t 
   return ZP_OK;
 }
 
+int zp_transcript_absorb_entries(zp_transcript* t, const uint8_t* bytes, uint64_t entry_bytes,
+                                 uint64_t count) {
+  if (!t || (count && entry_bytes && !bytes)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  for (uint64_t i = 0; i < count; i++) {
+    t->absorb_tag(0x6);
+    t->absorb(bytes + i * entry_bytes, entry_bytes);
+    t->absorb_tag(0x7);
+  }
+  return ZP_OK;
+}
+
 int zp_transcript_challenge(zp_transcript* t, uint8_t* h_out, uint32_t num_bytes) {
   if (!t || (num_bytes && !h_out)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
   t->challenge(h_out, num_bytes);

@@ -36,6 +36,7 @@ __all__ = [
     "ZipTypes", "BinPolyTypes", "IntTypes", "I64Types", "DensePolyTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
     "merkle_root", "lib", "set_option", "BatchCommit", "CommitServer", "PcsProverTranscript",
+    "step0_commit",
 ]
 
 
@@ -437,6 +438,15 @@ class PcsProverTranscript:
         buf = (ctypes.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
         _check(self._lib.zp_transcript_absorb_slice(self._h, buf, len(data)))
 
+    def absorb_entries(self, data, entry_bytes):
+        """absorb_slice of each consecutive entry of entry_bytes bytes."""
+        data = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8))
+        count = len(data) // entry_bytes if entry_bytes else 0
+        if count * entry_bytes != len(data):
+            raise ShapeError(L.ZP_ERR_SHAPE, "data is not a whole number of entries")
+        _check(self._lib.zp_transcript_absorb_entries(
+            self._h, data.ctypes.data_as(ctypes.c_void_p), entry_bytes, count))
+
     def get_challenge(self, num_bytes):
         """get_challenge::<T> for T of num_bytes bytes: its little-endian bytes."""
         buf = (ctypes.c_uint8 * max(1, num_bytes))()
@@ -467,6 +477,34 @@ class PcsProverTranscript:
         buf = (ctypes.c_uint8 * max(1, n))()
         _check(self._lib.zp_transcript_stream(self._h, buf, n))
         return bytes(buf)[:n]
+
+
+def step0_commit(params, witness, public):
+    """ZincPlusPiop::step0_commit (protocol/src/prover.rs:218-262).
+
+    params: (pp, types) for the binary, arbitrary and int traces;
+    witness / public: three lists of device tensors (columns, each one poly
+    of that trace's ZipTypes).  Each witness trace is committed as one batch
+    (commit_optionally, prover.rs:710-726: an empty trace gives no hint and
+    a zero root), the transcript starts from the three roots
+    (PcsProverTranscript::new_from_commitments) and absorbs every public
+    entry's transcription bytes (absorb_public_columns,
+    protocol/src/lib.rs:305-316).  -> (hints, commitments, transcript)."""
+    hints, comms = [], []
+    for (pp, types), cols in zip(params, witness):
+        if len(cols) == 0:
+            hints.append(None)
+            comms.append(ZipPlusCommitment(bytes(32), 0))
+            continue
+        h, c = ZipPlus(types).commit(pp, cols)
+        hints.append(h)
+        comms.append(c)
+    t = PcsProverTranscript.new_from_commitments(comms)
+    for (pp, types), cols in zip(params, public):
+        entry = 8 * types.eval_words if types.eval_type != "i32" else 4
+        for col in cols:
+            t.absorb_entries(col.contiguous().cpu().numpy().tobytes(), entry)
+    return hints, comms, t
 
 
 class ZipPlus:
