@@ -17,6 +17,7 @@
 //   proof-stream order (columns, Merkle proof headers and paths).
 // The host runs the transcript: alphas, the b elements, eval = <q_0, b>,
 // the coefficients s, the column indices, and the proof bytes.
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -45,21 +46,27 @@ eq_table_kernel(const Fp* __restrict__ rr, uint32_t nv, Fp* __restrict__ out, Fp
 
 constexpr int kSumWaves = 4;
 
-// One workgroup per (poly, row): P[row][c] = sum of q1[col] over the columns
-// whose eval has bit c set, reduced mod p.  Lane = (plane c, sub-column);
-// lanes of one wave read 64/Dp adjacent columns.
+// One workgroup per (poly row, block of columns): for every plane c the exact
+// sum (nine 32-bit words) of q1[col] over the block's columns whose eval has
+// bit c set.  Lane = (plane c, sub-column); lanes of one wave read 64/Dp
+// adjacent columns, four columns per lane in flight.
+// part[(row*nblk + blk)*D + c][9].
 template <typename Word>
 __global__ void __launch_bounds__(64 * kSumWaves)
-plane_sums_kernel(const Word* __restrict__ evals, const Fp* __restrict__ q1, Fp* __restrict__ out,
-                  uint64_t row_len, uint32_t D, uint32_t dp_log2, uint32_t red_bits, FpParams P) {
-  __shared__ uint32_t part[64 * kSumWaves][9];
+plane_sums_kernel(const Word* __restrict__ evals, const Fp* __restrict__ q1,
+                  uint32_t* __restrict__ part, uint64_t row_len, uint32_t cpb, uint32_t nblk,
+                  uint32_t D, uint32_t dp_log2) {
+  __shared__ uint32_t red[64 * kSumWaves][9];
   const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t Dp = 1u << dp_log2;
   const uint32_t c = lane & (Dp - 1), sub = lane >> dp_log2;
   const uint32_t per_step = (64u >> dp_log2) * kSumWaves;
-  const Word* x = evals + (uint64_t)blockIdx.x * row_len;
+  const uint64_t row = blockIdx.x / nblk;
+  const uint32_t blk = blockIdx.x % nblk;
+  const uint64_t c0 = (uint64_t)blk * cpb, c1 = min(row_len, c0 + cpb);
+  const Word* x = evals + row * row_len;
   uint32_t acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-  for (uint64_t col = (uint64_t)wave * (64u >> dp_log2) + sub; col < row_len; col += per_step) {
+  auto add = [&](uint64_t col) {
     const uint32_t mask = 0u - (uint32_t)((x[col] >> c) & 1u);
     const uint32_t* q = (const uint32_t*)(q1 + col);
     uint64_t carry = 0;
@@ -70,9 +77,15 @@ plane_sums_kernel(const Word* __restrict__ evals, const Fp* __restrict__ q1, Fp*
       carry = s >> 32;
     }
     acc[8] += (uint32_t)carry;
-  }
+  };
+  uint64_t col = c0 + (uint64_t)wave * (64u >> dp_log2) + sub;
+  for (; col + 3 * per_step < c1; col += 4 * per_step) {
 #pragma unroll
-  for (int k = 0; k < 9; k++) part[threadIdx.x][k] = acc[k];
+    for (int u = 0; u < 4; u++) add(col + u * per_step);
+  }
+  for (; col < c1; col += per_step) add(col);
+#pragma unroll
+  for (int k = 0; k < 9; k++) red[threadIdx.x][k] = acc[k];
   __syncthreads();
   if (threadIdx.x >= D) return;
   // Sum the partials of plane c: lanes c + m*Dp of every wave.
@@ -80,39 +93,62 @@ plane_sums_kernel(const Word* __restrict__ evals, const Fp* __restrict__ q1, Fp*
   for (uint32_t t = threadIdx.x; t < 64 * kSumWaves; t += Dp) {
     uint64_t carry = 0;
     for (int k = 0; k < 9; k++) {
-      const uint64_t v = (uint64_t)s[k] + part[t][k] + carry;
+      const uint64_t v = (uint64_t)s[k] + red[t][k] + carry;
       s[k] = (uint32_t)v;
       carry = v >> 32;
     }
   }
-  // s < 2^red_bits * p: subtract p * 2^k where it fits, k = red_bits-1 .. 0.
+  uint32_t* o = part + (((uint64_t)blockIdx.x) * D + threadIdx.x) * 9;
+  for (int k = 0; k < 9; k++) o[k] = s[k];
+}
+
+// One thread per (poly row, plane): adds the blocks' partial sums and
+// reduces mod p.  The total is below row_len * p <= 2^red_bits * p: p * 2^k
+// is subtracted where it fits, k = red_bits-1 .. 0.
+__global__ void __launch_bounds__(256)
+plane_reduce_kernel(const uint32_t* __restrict__ part, Fp* __restrict__ out, uint64_t rows,
+                    uint32_t nblk, uint32_t D, uint32_t red_bits, FpParams P) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows * D) return;
+  const uint64_t row = i / D;
+  const uint32_t c = (uint32_t)(i % D);
+  uint32_t s[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (uint32_t b = 0; b < nblk; b++) {
+    const uint32_t* q = part + ((row * nblk + b) * D + c) * 9;
+    uint64_t carry = 0;
+    for (int k = 0; k < 9; k++) {
+      const uint64_t v = (uint64_t)s[k] + q[k] + carry;
+      s[k] = (uint32_t)v;
+      carry = v >> 32;
+    }
+  }
   for (int k = (int)red_bits - 1; k >= 0; k--) {
     uint32_t pk[9];
     const uint32_t* pw = (const uint32_t*)P.p;
-    for (int i = 0; i < 9; i++) {
-      const uint32_t lo = i < 8 ? pw[i] : 0;
-      const uint32_t lo_prev = i > 0 ? pw[i - 1] : 0;
-      pk[i] = k == 0 ? lo : (lo << k) | (i > 0 ? lo_prev >> (32 - k) : 0);
+    for (int w = 0; w < 9; w++) {
+      const uint32_t lo = w < 8 ? pw[w] : 0;
+      const uint32_t lo_prev = w > 0 ? pw[w - 1] : 0;
+      pk[w] = k == 0 ? lo : (lo << k) | (w > 0 ? lo_prev >> (32 - k) : 0);
     }
     bool ge = true;
-    for (int i = 8; i >= 0; i--) {
-      if (s[i] != pk[i]) {
-        ge = s[i] > pk[i];
+    for (int w = 8; w >= 0; w--) {
+      if (s[w] != pk[w]) {
+        ge = s[w] > pk[w];
         break;
       }
     }
     if (ge) {
       uint64_t br = 0;
-      for (int i = 0; i < 9; i++) {
-        const uint64_t d = (uint64_t)s[i] - pk[i] - br;
-        s[i] = (uint32_t)d;
+      for (int w = 0; w < 9; w++) {
+        const uint64_t d = (uint64_t)s[w] - pk[w] - br;
+        s[w] = (uint32_t)d;
         br = (d >> 32) & 1;
       }
     }
   }
   Fp r;
-  for (int i = 0; i < 4; i++) r.w[i] = (uint64_t)s[2 * i] | ((uint64_t)s[2 * i + 1] << 32);
-  out[(uint64_t)blockIdx.x * D + threadIdx.x] = r;
+  for (int w = 0; w < 4; w++) r.w[w] = (uint64_t)s[2 * w] | ((uint64_t)s[2 * w + 1] << 32);
+  out[i] = r;
 }
 
 // b[j] = sum_b sum_c alpha_{b,c} * P[b][j][c] (all Montgomery form).
@@ -331,6 +367,13 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   bufs.add((void**)&d_q1, row_len * sizeof(Fp));
   bufs.add((void**)&d_q0, num_rows * sizeof(Fp));
   bufs.add((void**)&d_planes, rows * Dk * sizeof(Fp));
+  // Column blocks of the plane sums: at least 256 columns each, and enough
+  // blocks for ~2048 workgroups over all rows.
+  const uint32_t nblk = (uint32_t)std::max<uint64_t>(
+      1, std::min<uint64_t>(row_len / 256, (2048 + rows - 1) / rows));
+  const uint32_t cpb = (uint32_t)((row_len + nblk - 1) / nblk);
+  uint32_t* d_part;
+  bufs.add((void**)&d_part, rows * nblk * Dk * 9 * sizeof(uint32_t));
   bufs.add((void**)&d_b, num_rows * sizeof(Fp));
   bufs.add((void**)&d_alm, alm.size() * sizeof(Fp));
   bufs.add((void**)&d_al, al.size() * 8);
@@ -368,13 +411,15 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   const uint32_t dp_log2 = ceil_log2(Dk);
   const uint32_t red_bits = ceil_log2(row_len) + 1;
   if (binpoly)
-    hipLaunchKernelGGL(plane_sums_kernel<uint64_t>, dim3((uint32_t)rows), dim3(64 * kSumWaves), 0,
-                       s, (const uint64_t*)d_evals, d_q1, d_planes, row_len, Dk, dp_log2, red_bits,
-                       P);
+    hipLaunchKernelGGL(plane_sums_kernel<uint64_t>, dim3((uint32_t)(rows * nblk)),
+                       dim3(64 * kSumWaves), 0, s, (const uint64_t*)d_evals, d_q1, d_part, row_len,
+                       cpb, nblk, Dk, dp_log2);
   else
-    hipLaunchKernelGGL(plane_sums_kernel<uint32_t>, dim3((uint32_t)rows), dim3(64 * kSumWaves), 0,
-                       s, (const uint32_t*)d_evals, d_q1, d_planes, row_len, Dk, dp_log2, red_bits,
-                       P);
+    hipLaunchKernelGGL(plane_sums_kernel<uint32_t>, dim3((uint32_t)(rows * nblk)),
+                       dim3(64 * kSumWaves), 0, s, (const uint32_t*)d_evals, d_q1, d_part, row_len,
+                       cpb, nblk, Dk, dp_log2);
+  hipLaunchKernelGGL(plane_reduce_kernel, dim3((uint32_t)((rows * Dk + 255) / 256)), dim3(256), 0,
+                     s, d_part, d_planes, rows, nblk, Dk, red_bits, P);
   hipLaunchKernelGGL(b_kernel, dim3((uint32_t)((num_rows + 255) / 256)), dim3(256), 0, s,
                      d_planes, d_alm, d_b, num_rows, Dk, batch, P);
   if (hipGetLastError() != hipSuccess) return done(fail(ZP_ERR_DEVICE, "launch failed"));
