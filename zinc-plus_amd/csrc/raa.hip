@@ -261,13 +261,15 @@ int launch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t
 // Full-row kernel for the plain code with D = 64 or 32 planes.
 //
 // The first prefix sum of plane p is a count of set bits of y1, so it is
-// stored compactly: the y1 bit-vector of every plane (N bits) plus a 16-bit
-// checkpoint per 64 positions (the count before that word):
-//   a[m][p] = cp[p][m / 64] + popcount(bits[p][m / 64] & ones(m % 64 + 1)).
-// A workgroup holds all planes of one row (64 planes, N = 8192: 64 KiB of
-// bits + 16 KiB of checkpoints).  A checkpoint per 128 positions would let
-// two workgroups share a CU but costs a third lookup per rank, which
-// measured slower.  Phase 1 puts
+// stored compactly: the y1 bit-vector of every plane (N bits, in 32-bit
+// words) plus a 16-bit checkpoint per word (the count before that word):
+//   a[m][p] = cp[p][m / 32] + popcount(bits[p][m / 32] << (31 - m % 32)).
+// The shift keeps bits 0 .. m % 32 of the word, so a rank is one xor, one
+// shift and one bit count (which adds the checkpoint).  Plane rows are an
+// odd number of 32-bit words apart (bits) and of bank words apart
+// (checkpoints), so the 64 lanes of a lookup (one per plane, same position)
+// hit 64 different banks.  A workgroup holds all planes of one row (64
+// planes, N = 8192: 64 KiB of bits + 32 KiB of checkpoints).  Phase 1 puts
 // positions in lanes and turns each plane's bits into 64-bit words with one
 // ballot per plane; phase 2 puts planes in lanes, so each output position
 // (4*D bytes) is written by one store instruction.  perm2 is copied to LDS
@@ -276,6 +278,22 @@ int launch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t
 constexpr uint32_t kRowThreads = 512;
 constexpr uint32_t kRowWaves = kRowThreads / 64;
 
+// Word strides of one plane: bits (odd, in 32-bit words) and checkpoints
+// (an odd number of 32-bit bank words, i.e. 2 mod 4 in 16-bit entries).
+__host__ __device__ inline uint32_t raa_bits_stride(uint32_t N) { return N / 32 + 1; }
+__host__ __device__ inline uint32_t raa_cp_stride(uint32_t N) {
+  const uint32_t w = N / 32;
+  return (w % 4 == 0) ? w + 2 : ((w + 3) / 4) * 4 + 2;
+}
+
+// Counts the bits of `word` at positions 0 .. (m mod 32), plus `base`.
+__device__ __forceinline__ uint32_t rank_word(uint32_t word, uint32_t m, uint32_t base) {
+  uint32_t t;
+  // The shift uses the low 5 bits of m ^ 31, i.e. 31 - m % 32.
+  asm("v_lshlrev_b32 %0, %1, %2" : "=v"(t) : "v"(m ^ 31u), "v"(word));
+  return __popc(t) + base;
+}
+
 template <int D>
 __global__ void __launch_bounds__(kRowThreads)
 raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
@@ -283,14 +301,15 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
                uint32_t row_len, uint32_t N, uint64_t row0) {
   constexpr uint32_t SUB = 64 / D;           // position streams per wave
   constexpr uint32_t NS = kRowWaves * SUB;   // position streams per workgroup
-  extern __shared__ uint64_t lds64[];
-  const uint32_t W = N / 64;                 // 64-bit words per plane
-  const uint32_t WP = W + 1;                 // padded plane stride (words)
-  const uint32_t CP = W + 1;                 // padded checkpoint stride
-  uint64_t* bits = lds64;                                  // [D][WP]
-  uint16_t* cp = (uint16_t*)(lds64 + (size_t)D * WP);      // [D][CP]
-  uint32_t* part = (uint32_t*)(lds64 + (size_t)D * WP + ((size_t)D * CP + 3) / 4);
-  uint32_t* p2 = part + 2 * kRowThreads;                   // [N] perm2
+  extern __shared__ uint32_t lds32[];
+  const uint32_t W = N / 64;                 // 64-bit words (ballots) per plane
+  const uint32_t W32 = N / 32;               // 32-bit words per plane
+  const uint32_t WP = raa_bits_stride(N);
+  const uint32_t CP = raa_cp_stride(N);
+  uint32_t* bits = lds32;                                        // [D][WP]
+  uint16_t* cp = (uint16_t*)(lds32 + (size_t)D * WP);            // [D][CP]
+  uint32_t* part = lds32 + (size_t)D * WP + ((size_t)D * CP) / 2;
+  uint32_t* p2 = part + 2 * kRowThreads;                         // [N] perm2
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t row = row0 + blockIdx.x;
   const uint64_t* x = evals + row * row_len;
@@ -325,7 +344,10 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
           whi = (uint32_t)(m >> 32);
         }
       }
-      if (lane < (uint32_t)D) bits[lane * WP + blk] = ((uint64_t)whi << 32) | wlo;
+      if (lane < (uint32_t)D) {
+        bits[lane * WP + 2 * blk] = wlo;
+        bits[lane * WP + 2 * blk + 1] = whi;
+      }
     }
   }
   __syncthreads();
@@ -335,17 +357,17 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   {
     constexpr uint32_t SEGS = kRowThreads / D;
     const uint32_t p = tid % D, seg = tid / D;
-    const uint32_t ws = (W + SEGS - 1) / SEGS;
-    const uint32_t w0 = min(W, seg * ws), w1 = min(W, w0 + ws);
+    const uint32_t ws = (W32 + SEGS - 1) / SEGS;
+    const uint32_t w0 = min(W32, seg * ws), w1 = min(W32, w0 + ws);
     uint32_t cnt = 0;
-    for (uint32_t w = w0; w < w1; w++) cnt += __popcll(bits[p * WP + w]);
+    for (uint32_t w = w0; w < w1; w++) cnt += __popc(bits[p * WP + w]);
     part[tid] = cnt;
     __syncthreads();
     uint32_t before = 0;
     for (uint32_t sg = 0; sg < seg; sg++) before += part[sg * D + p];
     for (uint32_t w = w0; w < w1; w++) {
       cp[p * CP + w] = (uint16_t)before;
-      before += __popcll(bits[p * WP + w]);
+      before += __popc(bits[p * WP + w]);
     }
   }
   __syncthreads();
@@ -358,11 +380,11 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   const uint32_t s = wave * SUB + sub;
   constexpr uint32_t K = 64 / SUB;
   const uint32_t steps = N / (NS * K);
-  const uint64_t* bp = bits + p * WP;
+  const uint32_t* bp = bits + p * WP;
   const uint16_t* cpp = cp + p * CP;
   auto rank = [&](uint32_t m) -> uint32_t {
-    const uint64_t mask = (2ull << (m & 63)) - 1ull;  // ones(m % 64 + 1)
-    return cpp[m >> 6] + __popcll(bp[m >> 6] & mask);
+    const uint32_t w = m >> 5;
+    return rank_word(bp[w], m, cpp[w]);
   };
   int32_t* orow = out + row * N * D + p;
   uint32_t run = 0;  // every position of the earlier steps
@@ -402,9 +424,8 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
 constexpr uint32_t kRowPartWords = 2 * kRowThreads;
 
 size_t raa_row_lds(uint32_t D, uint32_t N) {
-  const size_t WP = N / 64 + 1, CP = N / 64 + 1;
-  return (size_t)D * WP * 8 + (((size_t)D * CP + 3) / 4) * 8 + (size_t)kRowPartWords * 4 +
-         (size_t)N * 4;
+  return (size_t)D * raa_bits_stride(N) * 4 + (size_t)D * raa_cp_stride(N) * 2 +
+         (size_t)kRowPartWords * 4 + (size_t)N * 4;
 }
 
 template <int D>
