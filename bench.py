@@ -220,6 +220,54 @@ def leg_prove(torch, zipgpu, synth, dev, cpu=True):
     return out
 
 
+def leg_step0(torch, zipgpu, synth, dev, cpu=True):
+    """The protocol's step 0 (protocol/src/prover.rs:218-262) at its IPRS
+    set-up (one row per poly, row_len = poly size 2^14): 4 binary
+    (BinaryPoly<32>), 2 arbitrary (DensePolynomial<i64, 32>) and 4 int (i64)
+    witness columns committed, one public column of each absorbed."""
+    import numpy as np
+    n, D = 1 << 14, 32
+    groups = [("binpoly", zipgpu.BinPolyTypes(D, 8), 4), ("dense_i64", zipgpu.DensePolyTypes(D), 2),
+              ("i64", zipgpu.I64Types(), 4)]
+    code = zipgpu.IprsCode.new_with_optimal_depth(n)
+    params, wit, pub, host = [], [], [], []
+    for g, (et, types, cols) in enumerate(groups):
+        pp = zipgpu.ZipPlus(types).setup(n, code)
+        params.append((pp, types))
+        xs = []
+        for k in range(cols + 1):
+            if et == "binpoly":
+                xs.append(synth.binary_polys(300 + 10 * g + k, n, D))
+            else:
+                xs.append((synth.splitmix64(300 + 10 * g + k, n * types.eval_words)
+                           .view(np.int64) >> np.int64(40)))
+        host.append(xs)
+        pub.append([torch.from_numpy(xs[0]).to(dev)])
+        wit.append([torch.from_numpy(v).to(dev) for v in xs[1:]])
+
+    def step():
+        hints, _, _ = zipgpu.step0_commit(params, wit, pub)
+        for h in hints:
+            h.free()
+    ms = wall_ms(torch, step, 10, 2)
+    evals = n * sum(c for _, _, c in groups)
+    out = {"workload": "step0_commit: 4 BinaryPoly<32> + 2 DensePolynomial<i64,32> + 4 i64 "
+                       "witness columns of 2^14 evals, IPRS row 2^14, public columns absorbed",
+           "step0_ms": ms, "witness_evals_per_s": evals / ms * 1e3}
+    if cpu:
+        from oracle import oracle
+        oracle.set_threads(min(16, os.cpu_count() or 1))
+        t0 = time.perf_counter()
+        for g, (et, types, cols) in enumerate(groups):
+            xs = np.concatenate(host[g][1:])
+            if et == "binpoly":
+                oracle.commit("iprs", xs.view(np.uint64), 1, n, batch=cols, D=D)
+            else:
+                oracle.commit_i64("iprs", xs, 1, n, cols, types.degree_plus_one, et)
+        out["cpu_oracle_commits_ms"] = (time.perf_counter() - t0) * 1e3
+    return out
+
+
 def leg_batched(torch, zipgpu, synth, dev):
     """64 commits, log2 n uniform in 12..20, D 32/64, IPRS/RAA: one batch plan
     (HIP graph, autotuned variants) against one zp_commit per poly."""
@@ -488,7 +536,8 @@ def main():
         for name, fn in (("plain_zip_2p16_i32", leg_plain_zip), ("raa_2p22_d64", leg_raa),
                          ("batched_64_mixed", leg_batched), ("served_commit_open", leg_served),
                          ("encode_wide_int5", leg_encode_wide),
-                         ("prove", lambda *a: leg_prove(*a, cpu=not args.no_cpu))):
+                         ("prove", lambda *a: leg_prove(*a, cpu=not args.no_cpu)),
+                         ("step0_commit", lambda *a: leg_step0(*a, cpu=not args.no_cpu))):
             if args.legs and name not in args.legs.split(","):
                 continue
             log(f"leg {name} ...")
