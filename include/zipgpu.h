@@ -307,6 +307,16 @@ int zp_transcript_absorb_slice(zp_transcript* t, const uint8_t* bytes, uint64_t 
 int zp_transcript_absorb_entries(zp_transcript* t, const uint8_t* bytes, uint64_t entry_bytes,
                                  uint64_t count);
 
+/* zp_transcript_absorb_entries with the entries in device memory (the same
+ * interface, protocol/src/lib.rs:305-316, for columns that live in HBM):
+ * the same bytes are absorbed and the transcript ends in the same state.
+ * The full 1 KiB BLAKE3 chunks of the framed stream are hashed on the GPU
+ * and only their chaining values come back; small streams are copied and
+ * hashed on the host.  Enqueued on `stream` and synchronised before
+ * returning. */
+int zp_transcript_absorb_entries_device(zp_transcript* t, const void* d_bytes,
+                                        uint64_t entry_bytes, uint64_t count, zp_stream stream);
+
 /* Transcript::get_challenge::<T> with T::NUM_BYTES = num_bytes: the
  * little-endian bytes of T (lib.rs:48-55). */
 int zp_transcript_challenge(zp_transcript* t, uint8_t* h_out, uint32_t num_bytes);

@@ -161,6 +161,11 @@ int launch_chunk_cvs(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32
                      hipStream_t s);
 int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
                     uint8_t* leaves, hipStream_t s);
+// Chaining values (8 words each) of n full chunks of the stream that frames
+// `entry_bytes`-byte entries of `data` as 0x6 || entry || 0x7, starting at
+// stream offset q0 with chunk counter c0.
+int launch_framed_chunk_cvs(const uint8_t* data, uint64_t entry_bytes, uint64_t q0, uint64_t c0,
+                            uint64_t n, uint32_t* cvs, hipStream_t s);
 // A second stream (with events) per device for overlapping commit stages.
 int aux_stream(hipStream_t* out);
 int launch_merkle_layers(uint8_t* layers, uint64_t n, hipStream_t s);

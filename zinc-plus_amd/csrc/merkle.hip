@@ -298,7 +298,58 @@ merkle_tree_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32
   }
 }
 
+// Chunk chaining values of a transcript's framed byte stream: `count`
+// entries of E bytes absorbed as 0x6 || entry || 0x7 each
+// (zip-plus/src/pcs_transcript.rs absorb of public columns through
+// absorb_slice, transcript/src/traits.rs:388-392).  Thread k hashes the
+// chunk at stream offset q0 + 1024 k with chunk counter c0 + k; the caller
+// guarantees that every such chunk lies before the end of the stream (so it
+// is never the root chunk).
+__global__ void __launch_bounds__(256)
+framed_chunk_cv_kernel(const uint8_t* __restrict__ data, uint64_t E, uint64_t q0, uint64_t c0,
+                       uint64_t n, uint32_t* __restrict__ cvs) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const uint64_t q = q0 + 1024 * k;
+  uint64_t i = q / (E + 2), u = q % (E + 2);  // entry, offset in its framed form
+  uint32_t cv[8];
+#pragma unroll
+  for (int w = 0; w < 8; w++) cv[w] = kIV[w];
+  for (uint32_t b = 0; b < 16; b++) {
+    uint32_t m[16];
+#pragma unroll
+    for (int w = 0; w < 16; w++) {
+      uint32_t word = 0;
+#pragma unroll
+      for (int y = 0; y < 4; y++) {
+        const uint32_t v = u == 0 ? 0x6u : u == E + 1 ? 0x7u : (uint32_t)data[i * E + u - 1];
+        word |= v << (8 * y);
+        if (++u == E + 2) {
+          u = 0;
+          i++;
+        }
+      }
+      m[w] = word;
+    }
+    compress(cv, m, c0 + k, 64, (b == 0 ? kChunkStart : 0u) | (b == 15 ? kChunkEnd : 0u));
+  }
+  uint4* dst = (uint4*)(cvs + k * 8);
+  dst[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
+  dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
+}
+
 }  // namespace
+
+int launch_framed_chunk_cvs(const uint8_t* data, uint64_t entry_bytes, uint64_t q0, uint64_t c0,
+                            uint64_t n, uint32_t* cvs, hipStream_t s) {
+  if (n == 0) return ZP_OK;
+  const uint64_t blocks = (n + 255) / 256;
+  if (blocks > 0x7fffffffull) return fail(ZP_ERR_INVALID_PARAM, "stream too long");
+  hipLaunchKernelGGL(framed_chunk_cv_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, data,
+                     entry_bytes, q0, c0, n, cvs);
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
+}
 
 uint64_t leaf_chunks(uint64_t nrows, uint32_t elem) {
   const uint64_t total = nrows * elem;

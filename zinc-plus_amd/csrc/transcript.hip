@@ -151,6 +151,28 @@ class Blake3 {
     }
   }
 
+  // Bytes absorbed so far.
+  uint64_t total_len() const { return chunk_ * 1024 + chunk_len(); }
+  size_t chunk_len() const { return (size_t)blocks_ * 64 + buf_len_; }
+  uint64_t chunk_index() const { return chunk_; }
+
+  // With the current chunk full and more input to follow: compress it and
+  // push its chaining value, leaving an empty chunk (what update does at the
+  // next byte).
+  void flush_full_chunk() {
+    uint32_t cv[8];
+    chunk_node().chaining(cv);
+    push_chunk(cv, chunk_ + 1);
+    reset_chunk(chunk_ + 1);
+  }
+
+  // With an empty current chunk and more input to follow: take the chaining
+  // value of the full chunk at this chunk index, computed elsewhere.
+  void push_chunk_cv(const uint32_t cv[8]) {
+    push_chunk(cv, chunk_ + 1);
+    reset_chunk(chunk_ + 1);
+  }
+
  private:
   struct Cv {
     uint32_t w[8];
@@ -160,8 +182,6 @@ class Blake3 {
   uint8_t buf_[64];
   uint32_t buf_len_ = 0, blocks_ = 0;
   std::vector<Cv> stack_;
-
-  size_t chunk_len() const { return (size_t)blocks_ * 64 + buf_len_; }
 
   void reset_chunk(uint64_t idx) {
     std::memcpy(cv_, kIV, 32);
@@ -376,6 +396,72 @@ int zp_transcript_absorb_entries(zp_transcript* t, const uint8_t* bytes, uint64_
     t->absorb_tag(0x7);
   }
   return ZP_OK;
+}
+
+namespace {
+
+// Absorbs bytes [qb, qe) of the framed stream of `count` entries of E bytes
+// (0x6 || entry || 0x7 each) whose data lies in device memory.
+int feed_framed(zp_transcript* t, const uint8_t* d, uint64_t E, uint64_t qb, uint64_t qe,
+                hipStream_t s) {
+  if (qe <= qb) return ZP_OK;
+  const uint64_t F = E + 2;
+  auto data_pos = [&](uint64_t q) {  // data bytes before stream offset q
+    const uint64_t i = q / F, u = q % F;
+    return i * E + (u == 0 ? 0 : std::min(u - 1, E));
+  };
+  const uint64_t lo = data_pos(qb), hi = data_pos(qe);
+  std::vector<uint8_t> hd(hi - lo), out(qe - qb);
+  if (hi > lo) {
+    ZP_HIP(hipMemcpyAsync(hd.data(), d + lo, hi - lo, hipMemcpyDeviceToHost, s));
+    ZP_HIP(hipStreamSynchronize(s));
+  }
+  uint64_t i = qb / F, u = qb % F;
+  for (uint64_t q = qb; q < qe; q++) {
+    out[q - qb] = u == 0 ? 0x6 : u == E + 1 ? 0x7 : hd[i * E + u - 1 - lo];
+    if (++u == F) {
+      u = 0;
+      i++;
+    }
+  }
+  t->absorb(out.data(), out.size());
+  return ZP_OK;
+}
+
+// Below this many stream bytes the device absorb copies the data and hashes
+// it on the host.
+constexpr uint64_t kDeviceAbsorbMin = 32 * 1024;
+
+}  // namespace
+
+int zp_transcript_absorb_entries_device(zp_transcript* t, const void* d_bytes,
+                                        uint64_t entry_bytes, uint64_t count, zp_stream zs) {
+  const hipStream_t stream = (hipStream_t)zs;
+  if (!t || (count && entry_bytes && !d_bytes)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  const uint64_t E = entry_bytes, F = E + 2;
+  if (F < E || (count && F > UINT64_MAX / count))
+    return fail(ZP_ERR_INVALID_PARAM, "stream too long");
+  const uint64_t T = count * F;
+  const uint8_t* d = (const uint8_t*)d_bytes;
+  if (T < kDeviceAbsorbMin) return feed_framed(t, d, E, 0, T, stream);
+  // Fill the current chunk on the host, then hash every further full chunk
+  // that is not the stream's last on the device; the host keeps the tail.
+  const uint64_t h = t->fs.total_len() == 0 ? 0 : 1024 - t->fs.chunk_len();
+  ZP_TRY(feed_framed(t, d, E, 0, h, stream));
+  if (t->fs.total_len() != 0) t->fs.flush_full_chunk();
+  const uint64_t n = (T - h - 1) / 1024;
+  std::vector<uint32_t> cvs(n * 8);
+  uint32_t* d_cvs = nullptr;
+  ZP_TRY(dev_alloc((void**)&d_cvs, n * 32, stream));
+  int rc = launch_framed_chunk_cvs(d, E, h, t->fs.chunk_index(), n, d_cvs, stream);
+  if (rc == ZP_OK && (hipMemcpyAsync(cvs.data(), d_cvs, n * 32, hipMemcpyDeviceToHost, stream) !=
+                          hipSuccess ||
+                      hipStreamSynchronize(stream) != hipSuccess))
+    rc = fail(ZP_ERR_DEVICE, "copy of chunk chaining values failed");
+  dev_free(d_cvs, stream);
+  if (rc != ZP_OK) return rc;
+  for (uint64_t k = 0; k < n; k++) t->fs.push_chunk_cv(&cvs[k * 8]);
+  return feed_framed(t, d, E, h + 1024 * n, T, stream);
 }
 
 int zp_transcript_challenge(zp_transcript* t, uint8_t* h_out, uint32_t num_bytes) {

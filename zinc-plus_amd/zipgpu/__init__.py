@@ -438,8 +438,21 @@ class PcsProverTranscript:
         buf = (ctypes.c_uint8 * max(1, len(data))).from_buffer_copy(data or b"\0")
         _check(self._lib.zp_transcript_absorb_slice(self._h, buf, len(data)))
 
-    def absorb_entries(self, data, entry_bytes):
-        """absorb_slice of each consecutive entry of entry_bytes bytes."""
+    def absorb_entries(self, data, entry_bytes, stream=None):
+        """absorb_slice of each consecutive entry of entry_bytes bytes.
+
+        `data` is bytes-like, or a device tensor: then its full BLAKE3 chunks
+        are hashed on the GPU (zp_transcript_absorb_entries_device), on
+        `stream` or the current stream."""
+        if isinstance(data, torch.Tensor) and data.is_cuda:
+            t = data.contiguous()
+            nbytes = t.numel() * t.element_size()
+            count = nbytes // entry_bytes if entry_bytes else 0
+            if count * entry_bytes != nbytes:
+                raise ShapeError(L.ZP_ERR_SHAPE, "data is not a whole number of entries")
+            _check(self._lib.zp_transcript_absorb_entries_device(
+                self._h, ctypes.c_void_p(t.data_ptr()), entry_bytes, count, _stream(stream)))
+            return
         data = np.ascontiguousarray(np.frombuffer(bytes(data), dtype=np.uint8))
         count = len(data) // entry_bytes if entry_bytes else 0
         if count * entry_bytes != len(data):
@@ -503,7 +516,7 @@ def step0_commit(params, witness, public):
     for (pp, types), cols in zip(params, public):
         entry = 8 * types.eval_words if types.eval_type != "i32" else 4
         for col in cols:
-            t.absorb_entries(col.contiguous().cpu().numpy().tobytes(), entry)
+            t.absorb_entries(col, entry)
     return hints, comms, t
 
 

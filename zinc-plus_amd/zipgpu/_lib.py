@@ -116,6 +116,7 @@ SIGNATURES = {
     "zp_transcript_destroy": (None, [_c_void_p]),
     "zp_transcript_absorb_slice": (_int, [_c_void_p, _c_void_p, _u64]),
     "zp_transcript_absorb_entries": (_int, [_c_void_p, _c_void_p, _u64, _u64]),
+    "zp_transcript_absorb_entries_device": (_int, [_c_void_p, _c_void_p, _u64, _u64, _c_void_p]),
     "zp_transcript_challenge": (_int, [_c_void_p, _c_void_p, _u32]),
     "zp_transcript_squeeze_idx": (_int, [_c_void_p, _u64, ctypes.POINTER(_u64)]),
     "zp_transcript_random_field_cfg": (_int, [_c_void_p, _c_void_p]),
