@@ -546,7 +546,9 @@ class ZipPlus:
         else:
             if len(polys) == 0:
                 raise ShapeError(L.ZP_ERR_SHAPE, "Batch must contain at least one polynomial")
-            t = torch.stack([p.reshape(-1) for p in polys])
+            # One poly: a view (no device copy); several: one contiguous batch.
+            t = (polys[0].reshape(1, -1) if len(polys) == 1
+                 else torch.stack([p.reshape(-1) for p in polys]))
         if t.shape[0] == 0:
             raise ShapeError(L.ZP_ERR_SHAPE, "Batch must contain at least one polynomial")
         expected = pp.num_rows * pp.linear_code.row_len() * self.types.eval_words
