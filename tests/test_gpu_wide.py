@@ -35,6 +35,31 @@ def test_iprs_encode_wide(zg, dev, oracle, row_len, depth, L, rows, small):
     assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("row_len,depth,L", [(1 << 14, 2, 1), (1 << 14, 2, 5), (256, 1, 8),
+                                              (2048, 2, 2), (1 << 13, 1, 3)])
+def test_iprs_wide_base_layers_agree(zg, dev, oracle, row_len, depth, L):
+    # The matrix-core base layer (signed byte digits x int8 slices) and the
+    # per-element base layer are separate device paths; rows of extreme
+    # limbs (all ones, sign bit only, alternating) and random rows.
+    say("wide_base", row_len, depth, L)
+    rows = 4
+    x = synth.comb_rows(77 + row_len + L, rows * row_len, L, 0).reshape(rows, row_len, L)
+    x[0] = np.uint64(0xffffffffffffffff).view(np.int64)
+    x[1] = 0
+    x[1, :, L - 1] = np.uint64(1 << 63).view(np.int64)
+    x[2, ::2] = np.uint64(0x7fffffffffffffff).view(np.int64)
+    code = zg.IprsCode(row_len, depth)
+    xd = to_dev(x, dev)
+    try:
+        zg.set_option("wide_base", 1)
+        generic = limbs(code.encode_wide(xd))
+    finally:
+        zg.set_option("wide_base", -1)
+    fast = limbs(code.encode_wide(xd))
+    want = oracle.iprs_encode_wide(x.view(np.uint64), row_len, depth, L)
+    assert np.array_equal(generic, want) and np.array_equal(fast, want)
+
+
 @pytest.mark.parametrize("row_len,L,flip,rows,small", [
     (2048, 5, False, 2, 2), (2048, 5, True, 2, 2), (256, 3, True, 3, 0), (8, 8, False, 4, 0),
     (1 << 13, 4, True, 1, 0), (16, 1, True, 5, 0), (1 << 11, 8, False, 1, 2),

@@ -166,6 +166,178 @@ wide_base_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
   if (r < bd) big_store(out + lay.out_at(rp, (uint64_t)c * bd + r), acc);
 }
 
+// ---- Base layer on the matrix cores (depth 1 or 2, bl <= 256) ----------
+//
+// out[c*bd + r] = sum_k f(r*k mod bd) x[o + k 8^d] is an integer matrix
+// product F (bd x bl) times X (bl x columns), columns n = (sequence, chunk).
+// F is held as three int8 slices (the binary encoder's A fragments:
+// f = s0 + 128 s1 + 16384 s2).  X is split into signed base-256 digits:
+// with C = 0x80 in every byte, the bytes y_b of y = x + C (modulo 2^(64L))
+// give d_b = y_b - 128 in [-128, 127] and x = sum_b d_b 256^b modulo
+// 2^(64L).  Every slice product F_a D_b is one run of
+// v_mfma_i32_16x16x64_i8 per 64 k (|sum| < 2^22), and
+//   out = sum_{a,b} 2^(7a + 8b) (F_a D_b)   modulo 2^(64L).
+// A one-limb input (|x| < 2^63, sign-extended) needs only 9 digits (8 for
+// L = 1); a full input needs 8L.
+typedef int wv4i __attribute__((ext_vector_type(4)));
+
+inline uint32_t wide_digit_count(bool one_limb, int L) {
+  return one_limb ? (L == 1 ? 8u : 9u) : 8u * (uint32_t)L;
+}
+
+// Thread per (column n, group of 16 k): writes the digit bytes of the 16
+// inputs x[n][16 kg + j], j < 16, in the B-operand layout: block
+// ((n / 16) * S + s) * nd + b holds 64 lanes x 16 bytes, lane 16 g + n % 16
+// with k = 64 s + 16 g + j at byte j (kg = 4 s + g).  Columns n >= ncols
+// and k >= bl are zero.
+template <int L>
+__global__ void __launch_bounds__(256)
+wide_digits_kernel(const uint64_t* __restrict__ in, uint8_t* __restrict__ frag, WideLayout lay,
+                   uint32_t bl, uint32_t depth, uint32_t nchunks, uint32_t S, uint32_t nd,
+                   uint64_t ncols, uint64_t total) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  const uint32_t nkg = 4 * S;
+  const uint64_t n = t / nkg;
+  const uint32_t kg = (uint32_t)(t % nkg), s = kg / 4, g = kg % 4;
+  const bool live = n < ncols;
+  const uint64_t rp = live ? n / nchunks : 0;
+  const uint32_t o = live ? octrev_w((uint32_t)(n % nchunks), depth) : 0;
+  constexpr uint64_t C = 0x8080808080808080ull;
+  uint64_t carry[16];
+#pragma unroll
+  for (int j = 0; j < 16; j++) carry[j] = 0;
+  uint8_t* dst = frag + (((n / 16) * S + s) * nd) * 1024 + (16 * g + n % 16) * 16;
+  for (uint32_t limb = 0; limb * 8 < nd; limb++) {
+    uint64_t y[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const uint32_t k = 16 * kg + j;
+      uint64_t x = 0;
+      if (live && k < bl) {
+        const uint64_t* p = in + lay.in_at(rp, (uint64_t)o + (uint64_t)k * nchunks);
+        x = lay.in_one_limb ? (limb == 0 ? p[0] : ((int64_t)p[0] < 0 ? ~0ull : 0ull)) : p[limb];
+      }
+      const uint64_t sum = x + C;
+      const uint64_t c1 = sum < x;
+      y[j] = sum + carry[j];
+      carry[j] = c1 | (y[j] < sum);
+    }
+    const uint32_t nb = min(8u, nd - limb * 8);
+    for (uint32_t bb = 0; bb < nb; bb++) {
+      uint32_t w[4];
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        w[q] = 0;
+#pragma unroll
+        for (int z = 0; z < 4; z++)
+          w[q] |= (uint32_t)(((y[4 * q + z] >> (8 * bb)) & 0xff) ^ 0x80) << (8 * z);
+      }
+      *(uint4*)(dst + (limb * 8 + bb) * 1024) = make_uint4(w[0], w[1], w[2], w[3]);
+    }
+  }
+}
+
+// Wave per 16 x 16 output tile (r-block rb, column tile ct); a workgroup is
+// four column tiles of one r-block.  Digits are taken four at a time (one
+// 32-bit word of the result): their slice products are added, shifted, into
+// a 64-bit column sum (each term is below 2^56, twelve of them fit), and
+// the column sums are carried into 32-bit result words in increasing order,
+// so the sign of every partial sum reaches the top word.
+template <int L, int S>
+__global__ void __launch_bounds__(256)
+wide_mfma_kernel(const wv4i* __restrict__ afrag, const wv4i* __restrict__ bfrag,
+                 uint64_t* __restrict__ out, WideLayout lay, uint32_t bd, uint32_t nd,
+                 uint32_t nchunks, uint64_t ncols, uint64_t nct, uint32_t nrb) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t rb = blockIdx.x % nrb;
+  const uint64_t ct = (uint64_t)(blockIdx.x / nrb) * 4 + wave;
+  if (ct >= nct) return;
+  wv4i A[S][3];
+#pragma unroll
+  for (int s = 0; s < S; s++)
+#pragma unroll
+    for (int a = 0; a < 3; a++) A[s][a] = afrag[(((uint64_t)rb * S + s) * 3 + a) * 64 + lane];
+  const wv4i* bt = bfrag + ct * S * nd * 64 + lane;
+  const uint64_t n = ct * 16 + (lane & 15);
+  const bool live = n < ncols;
+  const uint64_t rp = live ? n / nchunks : 0, c = live ? n % nchunks : 0;
+  uint32_t* dst[4];
+#pragma unroll
+  for (int v = 0; v < 4; v++) {
+    const uint32_t r = rb * 16 + 4 * (lane >> 4) + v;
+    dst[v] = (uint32_t*)(out + lay.out_at(rp, c * bd + r));
+  }
+  int64_t carry[4] = {0, 0, 0, 0};
+  for (uint32_t w = 0; w < 2 * (uint32_t)L; w++) {
+    int64_t col[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t bb = 0; bb < 4; bb++) {
+      const uint32_t b = 4 * w + bb;
+      if (b < nd) {
+        wv4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < S; s++) {
+          const wv4i B = bt[((uint64_t)s * nd + b) * 64];
+          acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][0], B, acc0, 0, 0, 0);
+          acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][1], B, acc1, 0, 0, 0);
+          acc2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][2], B, acc2, 0, 0, 0);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; v++)
+          col[v] += ((int64_t)acc0[v] << (8 * bb)) + ((int64_t)acc1[v] << (8 * bb + 7)) +
+                    ((int64_t)acc2[v] << (8 * bb + 14));
+      }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int64_t tot = col[v] + carry[v];
+      if (live) dst[v][w] = (uint32_t)tot;
+      carry[v] = tot >> 32;
+    }
+  }
+}
+
+template <int L, int S>
+int launch_wide_mfma_s(const zp_code* cd, const uint64_t* in, uint64_t seqs,
+                       const WideLayout& lay, uint64_t* out, hipStream_t s) {
+  const uint32_t nchunks = 1u << (3 * cd->depth);
+  const uint32_t nd = wide_digit_count(lay.in_one_limb, L);
+  const uint64_t ncols = seqs * nchunks, nct = (ncols + 15) / 16;
+  const uint64_t bytes = nct * S * nd * 1024;
+  uint8_t* frag = nullptr;
+  ZP_TRY(dev_alloc((void**)&frag, bytes, s));
+  const uint64_t total = nct * 16 * 4 * S;
+  const uint32_t nrb = cd->bd / 16;
+  const uint64_t blocks = nrb * ((nct + 3) / 4);
+  int rc = ZP_OK;
+  if ((total + 255) / 256 > 0x7fffffffull || blocks > 0x7fffffffull)
+    rc = fail(ZP_ERR_INVALID_PARAM, "too many rows");
+  if (rc == ZP_OK) {
+    hipLaunchKernelGGL((wide_digits_kernel<L>), dim3((uint32_t)((total + 255) / 256)), dim3(256),
+                       0, s, in, frag, lay, cd->bl, cd->depth, nchunks, (uint32_t)S, nd, ncols,
+                       total);
+    hipLaunchKernelGGL((wide_mfma_kernel<L, S>), dim3((uint32_t)blocks), dim3(256), 0, s,
+                       (const wv4i*)cd->d_afrag, (const wv4i*)frag, out, lay, cd->bd, nd, nchunks,
+                       ncols, nct, nrb);
+    if (hipGetLastError() != hipSuccess) rc = fail(ZP_ERR_DEVICE, "launch failed");
+  }
+  dev_free(frag, s);
+  return rc;
+}
+
+template <int L>
+int launch_wide_mfma(const zp_code* cd, const uint64_t* in, uint64_t seqs, const WideLayout& lay,
+                     uint64_t* out, hipStream_t s) {
+  switch (cd->mfma_ksteps) {
+    case 1: return launch_wide_mfma_s<L, 1>(cd, in, seqs, lay, out, s);
+    case 2: return launch_wide_mfma_s<L, 2>(cd, in, seqs, lay, out, s);
+    case 3: return launch_wide_mfma_s<L, 3>(cd, in, seqs, lay, out, s);
+    case 4: return launch_wide_mfma_s<L, 4>(cd, in, seqs, lay, out, s);
+  }
+  return fail(ZP_ERR_INVALID_PARAM, "unsupported base length for the matrix-core base layer");
+}
+
 // butterfly.rs:8-17.
 __constant__ uint8_t kBfly[8][7] = {
     {0, 0, 0, 0, 0, 0, 0}, {1, 2, 3, 4, 5, 6, 7}, {2, 4, 6, 0, 2, 4, 6}, {3, 6, 1, 4, 7, 2, 5},
@@ -270,10 +442,14 @@ int launch_wide(const zp_code* cd, const uint64_t* in, uint64_t seqs, const Wide
     const uint32_t nchunks = 1u << (3 * cd->depth);
     const uint32_t ntiles = (cd->bd + kWideTile - 1) / kWideTile;
     const uint64_t blocks = seqs * nchunks * ntiles;
-    if (blocks > 0x7fffffffull) return fail(ZP_ERR_INVALID_PARAM, "too many rows");
-    hipLaunchKernelGGL((wide_base_kernel<L>), dim3((uint32_t)blocks), dim3(kWideTile), 0, s, in,
-                       out, cd->d_ftab, lay, cd->bd, cd->bl, cd->depth, nchunks, ntiles);
-    ZP_HIP(hipGetLastError());
+    if (cd->d_afrag != nullptr && (cd->depth == 1 || cd->depth == 2) && g_force_wide != 1) {
+      ZP_TRY(launch_wide_mfma<L>(cd, in, seqs, lay, out, s));
+    } else {
+      if (blocks > 0x7fffffffull) return fail(ZP_ERR_INVALID_PARAM, "too many rows");
+      hipLaunchKernelGGL((wide_base_kernel<L>), dim3((uint32_t)blocks), dim3(kWideTile), 0, s,
+                         in, out, cd->d_ftab, lay, cd->bd, cd->bl, cd->depth, nchunks, ntiles);
+      ZP_HIP(hipGetLastError());
+    }
     for (uint32_t k = 0; k < cd->depth; k++) {
       const uint64_t sub = (uint64_t)cd->bd << (3 * k);
       const uint64_t total = seqs * (N / 8);
