@@ -298,39 +298,54 @@ merkle_tree_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32
   }
 }
 
-// Chunk chaining values of a transcript's framed byte stream: `count`
-// entries of E bytes absorbed as 0x6 || entry || 0x7 each
-// (zip-plus/src/pcs_transcript.rs absorb of public columns through
-// absorb_slice, transcript/src/traits.rs:388-392).  Thread k hashes the
-// chunk at stream offset q0 + 1024 k with chunk counter c0 + k; the caller
-// guarantees that every such chunk lies before the end of the stream (so it
-// is never the root chunk).
+// A transcript's framed byte stream: `count` entries of E bytes absorbed as
+// 0x6 || entry || 0x7 each (zip-plus/src/pcs_transcript.rs absorb of
+// public columns through absorb_slice, transcript/src/traits.rs:388-392).
+// Thread per 16 stream bytes [q0 + 16 t, q0 + 16 t + 16), written to
+// out[16 t ..]; the caller keeps the range inside the stream.
 __global__ void __launch_bounds__(256)
-framed_chunk_cv_kernel(const uint8_t* __restrict__ data, uint64_t E, uint64_t q0, uint64_t c0,
-                       uint64_t n, uint32_t* __restrict__ cvs) {
+framed_stream_kernel(const uint8_t* __restrict__ data, uint64_t E, uint64_t q0, uint64_t nwords,
+                     uint4* __restrict__ out) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nwords) return;
+  const uint64_t q = q0 + 16 * t;
+  uint64_t i = q / (E + 2), u = q % (E + 2);
+  uint32_t w[4];
+#pragma unroll
+  for (int z = 0; z < 4; z++) {
+    uint32_t word = 0;
+#pragma unroll
+    for (int y = 0; y < 4; y++) {
+      const uint32_t v = u == 0 ? 0x6u : u == E + 1 ? 0x7u : (uint32_t)data[i * E + u - 1];
+      word |= v << (8 * y);
+      if (++u == E + 2) {
+        u = 0;
+        i++;
+      }
+    }
+    w[z] = word;
+  }
+  out[t] = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// Chaining values of n full chunks of a byte stream (chunk k at stream +
+// 1024 k, counter c0 + k); none of them is the stream's last chunk.  The
+// next block's 64 bytes are loaded while the current one is compressed.
+__global__ void __launch_bounds__(64)
+stream_chunk_cv_kernel(const uint8_t* __restrict__ stream, uint64_t c0, uint64_t n,
+                       uint32_t* __restrict__ cvs) {
   const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= n) return;
-  const uint64_t q = q0 + 1024 * k;
-  uint64_t i = q / (E + 2), u = q % (E + 2);  // entry, offset in its framed form
+  const uint8_t* p = stream + 1024 * k;
   uint32_t cv[8];
 #pragma unroll
   for (int w = 0; w < 8; w++) cv[w] = kIV[w];
+  uint32_t m[16], nx[16];
+  load_block_words(p, nx);
   for (uint32_t b = 0; b < 16; b++) {
-    uint32_t m[16];
 #pragma unroll
-    for (int w = 0; w < 16; w++) {
-      uint32_t word = 0;
-#pragma unroll
-      for (int y = 0; y < 4; y++) {
-        const uint32_t v = u == 0 ? 0x6u : u == E + 1 ? 0x7u : (uint32_t)data[i * E + u - 1];
-        word |= v << (8 * y);
-        if (++u == E + 2) {
-          u = 0;
-          i++;
-        }
-      }
-      m[w] = word;
-    }
+    for (int w = 0; w < 16; w++) m[w] = nx[w];
+    if (b < 15) load_block_words(p + 64 * (b + 1), nx);
     compress(cv, m, c0 + k, 64, (b == 0 ? kChunkStart : 0u) | (b == 15 ? kChunkEnd : 0u));
   }
   uint4* dst = (uint4*)(cvs + k * 8);
@@ -342,13 +357,23 @@ framed_chunk_cv_kernel(const uint8_t* __restrict__ data, uint64_t E, uint64_t q0
 
 int launch_framed_chunk_cvs(const uint8_t* data, uint64_t entry_bytes, uint64_t q0, uint64_t c0,
                             uint64_t n, uint32_t* cvs, hipStream_t s) {
+  // In pieces of at most 2^16 chunks (64 MiB of staged stream).
+  constexpr uint64_t kPiece = 1ull << 16;
   if (n == 0) return ZP_OK;
-  const uint64_t blocks = (n + 255) / 256;
-  if (blocks > 0x7fffffffull) return fail(ZP_ERR_INVALID_PARAM, "stream too long");
-  hipLaunchKernelGGL(framed_chunk_cv_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, data,
-                     entry_bytes, q0, c0, n, cvs);
-  ZP_HIP(hipGetLastError());
-  return ZP_OK;
+  uint8_t* stage = nullptr;
+  ZP_TRY(dev_alloc((void**)&stage, std::min(n, kPiece) * 1024, s));
+  int rc = ZP_OK;
+  for (uint64_t k0 = 0; k0 < n && rc == ZP_OK; k0 += kPiece) {
+    const uint64_t nk = std::min(kPiece, n - k0);
+    const uint64_t nwords = nk * 64;
+    hipLaunchKernelGGL(framed_stream_kernel, dim3((uint32_t)((nwords + 255) / 256)), dim3(256), 0,
+                       s, data, entry_bytes, q0 + 1024 * k0, nwords, (uint4*)stage);
+    hipLaunchKernelGGL(stream_chunk_cv_kernel, dim3((uint32_t)((nk + 63) / 64)), dim3(64), 0, s,
+                       stage, c0 + k0, nk, cvs + k0 * 8);
+    if (hipGetLastError() != hipSuccess) rc = fail(ZP_ERR_DEVICE, "launch failed");
+  }
+  dev_free(stage, s);
+  return rc;
 }
 
 uint64_t leaf_chunks(uint64_t nrows, uint32_t elem) {

@@ -346,6 +346,8 @@ __constant__ uint8_t kBfly[8][7] = {
 
 // Stage k (radix8.rs:58-113): thread per (sequence, block, i); the 8 values
 // j*sub + i of a block of 8*sub are replaced by their radix-8 butterfly.
+// The plane of a sequence is the fastest thread index: the planes of one
+// element are adjacent in memory, so a wave's loads and stores coalesce.
 template <int L>
 __global__ void __launch_bounds__(256)
 wide_stage_kernel(uint64_t* __restrict__ data, const int32_t* __restrict__ tw, WideLayout lay,
@@ -353,7 +355,8 @@ wide_stage_kernel(uint64_t* __restrict__ data, const int32_t* __restrict__ tw, W
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const uint64_t per_row = N / 8;
-  const uint64_t rp = t / per_row, rem = t % per_row;
+  const uint64_t plane = t % lay.planes, rest = t / lay.planes;
+  const uint64_t rem = rest % per_row, rp = (rest / per_row) * lay.planes + plane;
   const uint64_t blk = rem / sub, i = rem % sub;
   const uint64_t first = blk * 8 * sub + i;
   Big<L> xs[8];
