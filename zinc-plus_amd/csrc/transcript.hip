@@ -14,6 +14,8 @@
 // The hasher is an incremental BLAKE3 (the published specification: 1 KiB
 // chunks, a stack of chunk chaining values merged as chunks complete, the
 // root node recompressed with increasing counters for extended output).
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -274,9 +276,12 @@ void par_memcpy(void* dst, const void* src, size_t n) {
   for (auto& x : th) x.join();
 }
 
-// The proof stream's bytes: grown with realloc (large blocks are remapped,
-// not copied).
+// The proof stream's bytes.  Small streams grow with realloc.  From 4 MiB
+// on (a proof with column openings reaches tens of MiB) the storage is
+// 2 MiB-aligned and advised for transparent huge pages: the first write of
+// a fresh 77 MB stream costs ~19,000 page faults with 4 KiB pages.
 struct ByteBuf {
+  static constexpr size_t kHuge = 2u << 20, kBig = 4u << 20;
   uint8_t* p = nullptr;
   size_t n = 0, cap = 0;
   ByteBuf() = default;
@@ -289,11 +294,22 @@ struct ByteBuf {
   ~ByteBuf() { std::free(p); }
   void reserve(size_t want) {
     if (want <= cap) return;
-    const size_t c = std::max(want, cap * 2);
-    uint8_t* q = (uint8_t*)std::realloc(p, std::max<size_t>(c, 64));
-    if (!q) throw std::bad_alloc();
-    p = q;
-    cap = std::max<size_t>(c, 64);
+    size_t c = std::max(want, cap * 2);
+    if (c < kBig) {
+      uint8_t* q = (uint8_t*)std::realloc(p, std::max<size_t>(c, 64));
+      if (!q) throw std::bad_alloc();
+      p = q;
+      cap = std::max<size_t>(c, 64);
+      return;
+    }
+    c = (c + kHuge - 1) & ~(kHuge - 1);
+    void* q = nullptr;
+    if (posix_memalign(&q, kHuge, c) != 0) throw std::bad_alloc();
+    (void)madvise(q, c, MADV_HUGEPAGE);
+    if (n) par_memcpy(q, p, n);
+    std::free(p);
+    p = (uint8_t*)q;
+    cap = c;
   }
   // Appends k uninitialised bytes and returns where they start.
   uint8_t* grow(size_t k) {
