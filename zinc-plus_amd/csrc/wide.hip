@@ -268,9 +268,28 @@ wide_mfma_kernel(const wv4i* __restrict__ afrag, const wv4i* __restrict__ bfrag,
     const uint32_t r = rb * 16 + 4 * (lane >> 4) + v;
     dst[v] = (uint32_t*)(out + lay.out_at(rp, c * bd + r));
   }
+  // The B fragments of the next four digits are loaded while the current
+  // four are multiplied (with few waves per SIMD, nothing else hides them).
+  auto load_word = [&](uint32_t w, wv4i (&Bw)[4][S]) {
+#pragma unroll
+    for (uint32_t bb = 0; bb < 4; bb++) {
+      const uint32_t b = 4 * w + bb;
+#pragma unroll
+      for (int s = 0; s < S; s++)
+        Bw[bb][s] = b < nd ? bt[((uint64_t)s * nd + b) * 64] : wv4i{0, 0, 0, 0};
+    }
+  };
+  wv4i Bn[4][S];
+  load_word(0, Bn);
   int64_t carry[4] = {0, 0, 0, 0};
   for (uint32_t w = 0; w < 2 * (uint32_t)L; w++) {
     int64_t col[4] = {0, 0, 0, 0};
+    wv4i Bc[4][S];
+#pragma unroll
+    for (uint32_t bb = 0; bb < 4; bb++)
+#pragma unroll
+      for (int s = 0; s < S; s++) Bc[bb][s] = Bn[bb][s];
+    if (w + 1 < 2 * (uint32_t)L && 4 * (w + 1) < nd) load_word(w + 1, Bn);
 #pragma unroll
     for (uint32_t bb = 0; bb < 4; bb++) {
       const uint32_t b = 4 * w + bb;
@@ -278,7 +297,7 @@ wide_mfma_kernel(const wv4i* __restrict__ afrag, const wv4i* __restrict__ bfrag,
         wv4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
 #pragma unroll
         for (int s = 0; s < S; s++) {
-          const wv4i B = bt[((uint64_t)s * nd + b) * 64];
+          const wv4i B = Bc[bb][s];
           acc0 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][0], B, acc0, 0, 0, 0);
           acc1 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][1], B, acc1, 0, 0, 0);
           acc2 = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[s][2], B, acc2, 0, 0, 0);
