@@ -194,3 +194,36 @@ def test_batch_autotune_flag(zg, dev, oracle):
     want = oracle_roots(oracle, meta, xs)
     for _ in range(3):
         assert [c.root for c in plan.commit(xd)] == want
+
+
+def test_batch_i64_and_dense_items_match_plain_commits(zg, dev):
+    # The i64 and DensePolynomial<i64, D> items (the protocol's int and
+    # arbitrary traces) encode through the Int<L> kernels, whose scratch
+    # (digit fragments, the RAA middle row) comes from the capture arena in
+    # a graph.  Their roots equal plain commits (themselves checked against
+    # the oracle in test_gpu_i64.py) on capture and on every replay.
+    say("batch i64/dense")
+    rng = np.random.default_rng(41)
+    specs = [("iprs", zg.I64Types(), 1 << 14, 2), ("iprs", zg.DensePolyTypes(16), 1 << 12, 1),
+             ("raa", zg.I64Types(), 1 << 12, 1), ("raa", zg.DensePolyTypes(8), 1 << 10, 2),
+             ("iprs", zg.BinPolyTypes(32), 1 << 12, 1)]
+    items, xs = [], []
+    for kind, types, n, batch in specs:
+        row_len = min(n, 1 << 12) if kind == "iprs" else synth.row_split("raa", n)
+        code = make_code(zg, kind, row_len)
+        items.append((zg.ZipPlus(types).setup(n, code), types, batch))
+        words = batch * n * types.eval_words
+        if types.eval_type == "binpoly":
+            xs.append(to_dev(synth.binary_polys(7, batch * n, 32), dev))
+        else:
+            xs.append(to_dev(rng.integers(-2**40, 2**40, words, dtype=np.int64), dev))
+    want = []
+    for (pp, types, batch), x in zip(items, xs):
+        polys = list(x.reshape(batch, -1))
+        h, c = zg.ZipPlus(types).commit(pp, polys)
+        h.free()
+        want.append(c.root)
+    plan = zg.BatchCommit(items, graph=True)
+    for rep in range(3):
+        assert [c.root for c in plan.commit(xs)] == want, rep
+        eager_noise(zg, dev)
