@@ -79,6 +79,8 @@ const char* zp_last_error(void);
  *   "wide_base":    0 = IPRS base layer of Int<L> / i64 rows on the matrix
  *                   cores where the code has its int8 tables (default),
  *                   1 = the per-element base layer.
+ *   "merkle_tree":  0 = Merkle layers with one 4-lane quad per parent
+ *                   (default), 1 = one thread per parent.
  * Returns ZP_ERR_INVALID_PARAM for an unknown name or value. */
 int zp_set_option(const char* name, int64_t value);
 

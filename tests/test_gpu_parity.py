@@ -323,3 +323,32 @@ def test_combined_row(zg, dev, oracle, rows, row_len, D, batch, L):
     got = zp.combined_row(pp, polys, to_dev(alphas, dev), to_dev(coeffs, dev)).cpu().numpy()
     want = oracle.combined_row(x.view(np.uint64), alphas, coeffs, rows, row_len, batch, D, L)
     assert np.array_equal(got.view(np.uint64), want)
+
+
+@pytest.mark.parametrize("kind,n,row_len,D", [("iprs", 1 << 6, 8, 32), ("iprs", 1 << 8, 64, 32),
+                                              ("raa", 1 << 12, 1 << 11, 32),
+                                              ("iprs", 1 << 16, 1 << 14, 32),
+                                              ("raa", 1 << 13, 1 << 13, 16)])
+def test_merkle_layer_kernels_agree(zg, dev, oracle, kind, n, row_len, D):
+    # Every layer of the tree (the proofs read them all), from the
+    # quad-per-node kernel and the thread-per-node kernel; 32 to 2^16 leaves.
+    say("merkle layers", kind, n, row_len, D)
+    x = synth.binary_polys(31 + n, n, D)
+    code = zg.IprsCode(row_len, -1) if kind == "iprs" else zg.RaaCode(row_len)
+    zp = zg.ZipPlus(zg.BinPolyTypes(D))
+    pp = zp.setup(n, code)
+    layers = {}
+    try:
+        for opt in (1, 0):
+            zg.set_option("merkle_tree", opt)
+            hint, comm = zp.commit(pp, [to_dev(x, dev)])
+            layers[opt] = ([hint.layer(lv) for lv in range(hint.height)], comm.root)
+            hint.free()
+    finally:
+        zg.set_option("merkle_tree", -1)
+    assert layers[0][1] == layers[1][1]
+    assert len(layers[0][0]) == len(layers[1][0]) and len(layers[0][0]) > 1
+    for a, b in zip(layers[0][0], layers[1][0]):
+        assert np.array_equal(a, b)
+    want = oracle.commit(kind, x.view(np.uint64), pp.num_rows, row_len, D=D)["root"]
+    assert layers[0][1] == want

@@ -257,6 +257,97 @@ uint32_t cv_tree_columns(uint64_t nchunks) {
   return (uint32_t)std::min<uint64_t>(g, 64);
 }
 
+// One BLAKE3 compression spread over the 4 lanes of a quad: lane q holds
+// column q of the state (v[q], v[4+q], v[8+q], v[12+q]) and runs that
+// column's G; for the diagonal step b, c and d are rotated across the quad
+// by 1, 2 and 3 lanes (DPP quad_perm), so lane q then holds
+// (v[q], v[4+(q+1)%4], v[8+(q+2)%4], v[12+(q+3)%4]), and rotated back.
+// Lane q's message words, in use order, are m[kQuadSched[q][i]]: per round,
+// the column step takes words s[2q], s[2q+1] of the round's schedule and the
+// diagonal step s[8+2q], s[9+2q].
+struct QuadSchedule {
+  uint8_t s[4][28];
+};
+constexpr QuadSchedule make_quad_schedule() {
+  QuadSchedule qs{};
+  for (int q = 0; q < 4; q++)
+    for (int r = 0; r < 7; r++) {
+      qs.s[q][4 * r + 0] = (uint8_t)kSched.s[r][2 * q];
+      qs.s[q][4 * r + 1] = (uint8_t)kSched.s[r][2 * q + 1];
+      qs.s[q][4 * r + 2] = (uint8_t)kSched.s[r][8 + 2 * q];
+      qs.s[q][4 * r + 3] = (uint8_t)kSched.s[r][9 + 2 * q];
+    }
+  return qs;
+}
+__device__ __constant__ QuadSchedule kQuadSched = make_quad_schedule();
+
+// quad_perm [1,2,3,0], [2,3,0,1], [3,0,1,2]: lane q takes lane (q + k) % 4.
+constexpr int kQuadRot1 = 0x39, kQuadRot2 = 0x4e, kQuadRot3 = 0x93;
+template <int kCtrl>
+__device__ __forceinline__ uint32_t quad_rot(uint32_t x) {
+  return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, kCtrl, 0xf, 0xf, false);
+}
+
+// Parent node of the 16 message words at msg (left CV, right CV), computed
+// by the quad; lane q returns output words q (lo) and 4 + q (hi).
+__device__ __forceinline__ void parent_quad(const uint32_t* msg, uint32_t q, bool root,
+                                            uint32_t& lo, uint32_t& hi) {
+  uint32_t m[28];
+#pragma unroll
+  for (int i = 0; i < 28; i++) m[i] = msg[kQuadSched.s[q][i]];
+  uint32_t a = kIV[q], b = kIV[4 + q], c = kIV[q];
+  uint32_t d = q < 2 ? 0u : q == 2 ? 64u : (kParent | (root ? kRoot : 0u));
+#pragma unroll
+  for (int r = 0; r < 7; r++) {
+    B3G(a, b, c, d, m[4 * r], m[4 * r + 1])
+    b = quad_rot<kQuadRot1>(b);
+    c = quad_rot<kQuadRot2>(c);
+    d = quad_rot<kQuadRot3>(d);
+    B3G(a, b, c, d, m[4 * r + 2], m[4 * r + 3])
+    b = quad_rot<kQuadRot3>(b);
+    c = quad_rot<kQuadRot2>(c);
+    d = quad_rot<kQuadRot1>(d);
+  }
+  lo = a ^ c;
+  hi = b ^ d;
+}
+
+// Merkle layers (merkle.rs:172-221) for n = 2^h leaves stored at layers[0..n),
+// layer L at node offset 2n - 2(n >> L).  A workgroup of 512 threads (128
+// quads, one parent per quad) reads 256 nodes of layer L0 and produces up
+// to 8 layers above them in LDS, writing each node to its place; the layer
+// with one node uses the root merge.  Every level is one compression
+// deep, so a quad per node cuts the latency of the 16 levels of a 2^16-leaf
+// tree about threefold against a thread per node.
+__global__ void __launch_bounds__(512)
+merkle_quad_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32_t levels) {
+  __shared__ uint32_t buf[256 * 8];
+  const uint32_t t = threadIdx.x, quad = t >> 2, q = t & 3;
+  const uint64_t cnt0 = n >> L0;
+  const uint64_t base = (uint64_t)blockIdx.x * 256;
+  const uint32_t m = (uint32_t)min((uint64_t)256, cnt0 - base);
+  const uint32_t* src = (const uint32_t*)(layers + 32 * (2 * n - 2 * cnt0));
+  for (uint32_t i = t; i < m * 8; i += 512) buf[i] = src[base * 8 + i];
+  __syncthreads();
+  uint32_t cur = m;
+  for (uint32_t lv = 1; lv <= levels && cur > 1; lv++) {
+    const uint32_t npar = cur / 2;
+    const uint64_t cnt = n >> (L0 + lv);
+    uint32_t lo = 0, hi = 0;
+    if (quad < npar) parent_quad(buf + quad * 16, q, cnt == 1, lo, hi);
+    __syncthreads();
+    if (quad < npar) {
+      uint32_t* dst = (uint32_t*)(layers + 32 * (2 * n - 2 * cnt + (base >> lv) + quad));
+      buf[quad * 8 + q] = lo;
+      buf[quad * 8 + 4 + q] = hi;
+      dst[q] = lo;
+      dst[4 + q] = hi;
+    }
+    __syncthreads();
+    cur = npar;
+  }
+}
+
 // Merkle layers (merkle.rs:172-221) for n = 2^h leaves stored at layers[0..n).
 // Layer L starts at node offset 2n - 2(n >> L).  A workgroup of 256 threads
 // reads 512 nodes of layer L0 and produces up to 9 layers above them in LDS,
@@ -455,12 +546,21 @@ int launch_merkle_layers(uint8_t* layers, uint64_t n, hipStream_t s) {
   while ((n >> L) > 1) {
     const uint64_t cnt = n >> L;
     const uint32_t lg = 63 - (uint32_t)__builtin_clzll(cnt);
-    const uint32_t levels = std::min<uint32_t>(9, lg);
-    const uint64_t blocks = (cnt + 511) / 512;
-    hipLaunchKernelGGL(merkle_tree_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, layers, n, L,
-                       levels);
-    ZP_HIP(hipGetLastError());
-    L += levels;
+    if (g_force_tree == 1) {
+      const uint32_t levels = std::min<uint32_t>(9, lg);
+      const uint64_t blocks = (cnt + 511) / 512;
+      hipLaunchKernelGGL(merkle_tree_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, layers, n,
+                         L, levels);
+      ZP_HIP(hipGetLastError());
+      L += levels;
+    } else {
+      const uint32_t levels = std::min<uint32_t>(8, lg);
+      const uint64_t blocks = (cnt + 255) / 256;
+      hipLaunchKernelGGL(merkle_quad_kernel, dim3((uint32_t)blocks), dim3(512), 0, s, layers, n,
+                         L, levels);
+      ZP_HIP(hipGetLastError());
+      L += levels;
+    }
   }
   return ZP_OK;
 }
