@@ -76,9 +76,9 @@ const char* zp_last_error(void);
  *                       length 8192), 2 = plane groups (4 planes at 8192).
  *   "commit_overlap": 1 = hash finished row groups on a second stream while
  *                   later rows encode, 0 = encode, then hash (default).
- *   "wide_base":    0 = IPRS base layer of Int<L> / i64 rows on the matrix
- *                   cores where the code has its int8 tables (default),
- *                   1 = the per-element base layer.
+ *   "wide_base":    0 = IPRS base layer of Int<L>, i64 and i32 rows on the
+ *                   matrix cores where the code has its int8 tables
+ *                   (default), 1 = the per-element kernels.
  *   "merkle_tree":  0 = Merkle layers with one 4-lane quad per parent
  *                   (default), 1 = one thread per parent.
  * Returns ZP_ERR_INVALID_PARAM for an unknown name or value. */

@@ -74,12 +74,21 @@ def test_iprs_device_encoders_agree(zg, dev, oracle, row_len, depth, D):
 @pytest.mark.parametrize("row_len,depth,rows", [
     (256, 1, 2), (1 << 14, 2, 4), (2048, 3, 1), (128, 0, 1), (1 << 12, -1, 3)])
 def test_iprs_i32_encode(zg, dev, oracle, row_len, depth, rows):
+    # Depths 1 and 2 take the matrix-core base layer (5 signed digits per
+    # i32) by default; wide_base = 1 forces the per-element i128 kernel.
     say("iprs_i32", row_len, depth, rows)
     x = synth.i32_evals(200 + row_len, rows * row_len)
+    x[0] = -2**31
+    x[-1] = 2**31 - 1
     code = zg.IprsCode(row_len, depth)
-    got = code.encode_rows(to_dev(x, dev), rows, zg.IntTypes()).cpu().numpy()
     want = oracle.iprs_encode_i32(x, rows, row_len, depth)
-    assert np.array_equal(got, want)
+    try:
+        for opt in (-1, 1):
+            zg.set_option("wide_base", opt)
+            got = code.encode_rows(to_dev(x, dev), rows, zg.IntTypes()).cpu().numpy()
+            assert np.array_equal(got, want), opt
+    finally:
+        zg.set_option("wide_base", -1)
 
 
 def test_iprs_extreme_inputs(zg, dev, oracle):

@@ -139,6 +139,8 @@ int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void*
         return launch_iprs_binary_mfma(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
       return launch_iprs_binary(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
     }
+    if (wide_mfma_eligible(c))
+      return launch_encode_i32_mfma(c, (const int32_t*)evals, num_rows, out, s);
     return launch_iprs_i32(c, (const int32_t*)evals, num_rows, out, s);
   }
   return launch_raa_binary(c, (const uint64_t*)evals, num_rows, D, (int32_t*)out, s,

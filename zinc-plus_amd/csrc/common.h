@@ -127,6 +127,9 @@ int launch_iprs_binary_mfma(const zp_code* c, const uint64_t* evals, uint64_t ro
                             int64_t* out, hipStream_t s);
 int launch_iprs_i32(const zp_code* c, const int32_t* evals, uint64_t rows, void* out,
                     hipStream_t s);
+bool wide_mfma_eligible(const zp_code* c);
+int launch_encode_i32_mfma(const zp_code* c, const int32_t* evals, uint64_t rows, void* out,
+                           hipStream_t s);
 int launch_raa_binary(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
                       int32_t* out, hipStream_t s, int variant);
 bool raa_full_row_eligible(const zp_code* c, uint32_t D);

@@ -18,6 +18,14 @@
 #include "common.h"
 
 namespace zpi {
+
+// The base layer runs on the matrix cores when the code has the int8 slices
+// of its base matrix (depth 1 or 2, base length <= 256), unless the
+// per-element kernel is forced.
+bool wide_mfma_eligible(const zp_code* cd) {
+  return cd->d_afrag != nullptr && (cd->depth == 1 || cd->depth == 2) && g_force_wide != 1;
+}
+
 namespace {
 
 template <int L>
@@ -125,6 +133,9 @@ struct WideLayout {
   bool in_one_limb;
   uint64_t in_row, in_plane, in_elem;
   uint64_t out_row, out_plane, out_elem;
+  // Input elements are i32 (sign-extended; offsets in 4-byte units).  Only
+  // the matrix-core base layer reads such inputs.
+  bool in_i32 = false;
   __device__ __forceinline__ uint64_t in_at(uint64_t rp, uint64_t k) const {
     return (rp / planes) * in_row + (rp % planes) * in_plane + k * in_elem;
   }
@@ -181,8 +192,9 @@ wide_base_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
 // L = 1); a full input needs 8L.
 typedef int wv4i __attribute__((ext_vector_type(4)));
 
-inline uint32_t wide_digit_count(bool one_limb, int L) {
-  return one_limb ? (L == 1 ? 8u : 9u) : 8u * (uint32_t)L;
+inline uint32_t wide_digit_count(const WideLayout& lay, int L) {
+  if (lay.in_i32) return 5;  // |x| < 2^31: x + C stays below 2^40
+  return lay.in_one_limb ? (L == 1 ? 8u : 9u) : 8u * (uint32_t)L;
 }
 
 // Thread per (column n, group of 16 k): writes the digit bytes of the 16
@@ -215,8 +227,14 @@ wide_digits_kernel(const uint64_t* __restrict__ in, uint8_t* __restrict__ frag, 
       const uint32_t k = 16 * kg + j;
       uint64_t x = 0;
       if (live && k < bl) {
-        const uint64_t* p = in + lay.in_at(rp, (uint64_t)o + (uint64_t)k * nchunks);
-        x = lay.in_one_limb ? (limb == 0 ? p[0] : ((int64_t)p[0] < 0 ? ~0ull : 0ull)) : p[limb];
+        const uint64_t at = lay.in_at(rp, (uint64_t)o + (uint64_t)k * nchunks);
+        if (lay.in_i32) {
+          const int64_t v = ((const int32_t*)in)[at];
+          x = limb == 0 ? (uint64_t)v : (v < 0 ? ~0ull : 0ull);
+        } else {
+          const uint64_t* p = in + at;
+          x = lay.in_one_limb ? (limb == 0 ? p[0] : ((int64_t)p[0] < 0 ? ~0ull : 0ull)) : p[limb];
+        }
       }
       const uint64_t sum = x + C;
       const uint64_t c1 = sum < x;
@@ -321,7 +339,7 @@ template <int L, int S>
 int launch_wide_mfma_s(const zp_code* cd, const uint64_t* in, uint64_t seqs,
                        const WideLayout& lay, uint64_t* out, hipStream_t s) {
   const uint32_t nchunks = 1u << (3 * cd->depth);
-  const uint32_t nd = wide_digit_count(lay.in_one_limb, L);
+  const uint32_t nd = wide_digit_count(lay, L);
   const uint64_t ncols = seqs * nchunks, nct = (ncols + 15) / 16;
   const uint64_t bytes = nct * S * nd * 1024;
   uint8_t* frag = nullptr;
@@ -464,7 +482,7 @@ int launch_wide(const zp_code* cd, const uint64_t* in, uint64_t seqs, const Wide
     const uint32_t nchunks = 1u << (3 * cd->depth);
     const uint32_t ntiles = (cd->bd + kWideTile - 1) / kWideTile;
     const uint64_t blocks = seqs * nchunks * ntiles;
-    if (cd->d_afrag != nullptr && (cd->depth == 1 || cd->depth == 2) && g_force_wide != 1) {
+    if (wide_mfma_eligible(cd)) {
       ZP_TRY(launch_wide_mfma<L>(cd, in, seqs, lay, out, s));
     } else {
       if (blocks > 0x7fffffffull) return fail(ZP_ERR_INVALID_PARAM, "too many rows");
@@ -523,6 +541,17 @@ int launch_encode_wide(const zp_code* cd, uint32_t L, const uint64_t* in, uint64
 // DensePolynomial<i64, D> evals (D coefficient planes, each encoded as its
 // own sequence: Cw = [i64; D] under IPRS, L = 1, or [Int<4>; D] under RAA,
 // L = 4).  Inputs are sign-extended on load; outputs are the Cw layout.
+// The i32 -> i128 IPRS encoder (IntZipTypes: Eval = i32, Cw = i128) through
+// the matrix-core base layer (5 digits per input) and the 2-limb stages.
+int launch_encode_i32_mfma(const zp_code* cd, const int32_t* in, uint64_t rows, void* out,
+                           hipStream_t s) {
+  if (rows == 0) return ZP_OK;
+  if (!wide_mfma_eligible(cd)) return fail(ZP_ERR_INVALID_PARAM, "code has no matrix-core tables");
+  WideLayout lay{1, true, cd->row_len, 0, 1, cd->N * 2, 0, 2};
+  lay.in_i32 = true;
+  return dispatch_wide(cd, 2, (const uint64_t*)in, rows, lay, (uint64_t*)out, s);
+}
+
 int launch_encode_i64_planes(const zp_code* cd, uint32_t D, uint32_t L, const int64_t* in,
                              uint64_t rows, void* out, hipStream_t s) {
   if (rows == 0) return ZP_OK;

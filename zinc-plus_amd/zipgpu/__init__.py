@@ -78,8 +78,8 @@ def set_option(name, value):
 
     "iprs_encoder" (0 matrix-core, 1 VALU), "raa_kernel" (0 full-row,
     1 / 2 plane groups), "commit_overlap" (1 overlap encode and hash, 0 not:
-    the default), "wide_base" (Int<L> and i64 IPRS base layer: 0 matrix
-    cores where the code has the int8 tables, 1 per-element), "merkle_tree"
+    the default), "wide_base" (Int<L>, i64 and i32 IPRS base layer: 0
+    matrix cores where the code has the int8 tables, 1 per-element), "merkle_tree"
     (0 a 4-lane quad per parent node, 1 a thread per parent).
     """
     _check(L.load().zp_set_option(name.encode(), int(value)))
