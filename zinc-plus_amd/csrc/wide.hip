@@ -375,11 +375,6 @@ int launch_wide_mfma(const zp_code* cd, const uint64_t* in, uint64_t seqs, const
   return fail(ZP_ERR_INVALID_PARAM, "unsupported base length for the matrix-core base layer");
 }
 
-// butterfly.rs:8-17.
-__constant__ uint8_t kBfly[8][7] = {
-    {0, 0, 0, 0, 0, 0, 0}, {1, 2, 3, 4, 5, 6, 7}, {2, 4, 6, 0, 2, 4, 6}, {3, 6, 1, 4, 7, 2, 5},
-    {4, 0, 4, 0, 4, 0, 4}, {5, 2, 7, 4, 1, 6, 3}, {6, 4, 2, 0, 6, 4, 2}, {7, 6, 5, 4, 3, 2, 1},
-};
 
 // Stage k (radix8.rs:58-113): thread per (sequence, block, i); the 8 values
 // j*sub + i of a block of 8*sub are replaced by their radix-8 butterfly.
@@ -399,11 +394,19 @@ wide_stage_kernel(uint64_t* __restrict__ data, const int32_t* __restrict__ tw, W
   Big<L> xs[8];
 #pragma unroll
   for (int j = 0; j < 8; j++) xs[j] = big_load<L>(data + lay.out_at(rp, first + (uint64_t)j * sub));
+  // All 56 coefficients of this position are loaded at once (a stage has
+  // few threads, so one load round trip per output would be exposed); the
+  // coefficient of input j in output r is tw[(j-1)*8 + (r*j mod 8)]
+  // (butterfly.rs:8-17).
   const int32_t* twi = tw + i * 56;
+  int32_t T[56];
+#pragma unroll
+  for (int e = 0; e < 56; e++) T[e] = twi[e];
+#pragma unroll
   for (int r = 0; r < 8; r++) {
     Big<L> a = xs[0];
 #pragma unroll
-    for (int jm1 = 0; jm1 < 7; jm1++) big_mac(a, xs[jm1 + 1], twi[jm1 * 8 + kBfly[r][jm1]]);
+    for (int jm1 = 0; jm1 < 7; jm1++) big_mac(a, xs[jm1 + 1], T[jm1 * 8 + ((r * (jm1 + 1)) & 7)]);
     big_store(data + lay.out_at(rp, first + (uint64_t)r * sub), a);
   }
 }
