@@ -46,6 +46,29 @@ def test_incremental_hash_across_chunks(zgh, oracle, sizes):
     assert t.get_challenge(131) == o.challenge(131)
 
 
+@pytest.mark.parametrize("sizes", [[8190], [8191], [8192], [8193], [9000], [8192 * 3 + 5],
+                                   [1, 8192 * 2 + 1, 20000], [1021, 65536 + 7], [300000],
+                                   [1024 * 8 - 2, 1024 * 8 - 2]])
+def test_multi_chunk_absorbs(zgh, oracle, sizes):
+    # Long absorbs go eight chunks at a time (SIMD lanes) once the hasher
+    # sits at a chunk boundary with more than eight chunks left; the oracle
+    # hashes one block at a time.  Sizes straddle that threshold from every
+    # starting offset (each absorb_slice adds two tag bytes).
+    import numpy as np
+    rng = np.random.default_rng(sum(sizes))
+    t, o = zgh.PcsProverTranscript(), oracle.Transcript()
+    for n in sizes:
+        data = rng.integers(0, 256, n, dtype=np.uint8).tobytes()
+        t.absorb_slice(data)
+        o.absorb_slice(data)
+        assert t.get_challenge(32) == o.challenge(32), n
+    entries = rng.integers(0, 256, 256 * 4096, dtype=np.uint8).tobytes()
+    t.absorb_entries(entries, 256)
+    for i in range(4096):
+        o.absorb_slice(entries[256 * i:256 * (i + 1)])
+    assert t.get_challenge(64) == o.challenge(64)
+
+
 def test_random_field_cfg_matches(zgh, oracle):
     for seed in range(4):
         root = bytes((seed * 37 + i) % 256 for i in range(32))

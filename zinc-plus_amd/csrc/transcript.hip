@@ -14,6 +14,7 @@
 // The hasher is an incremental BLAKE3 (the published specification: 1 KiB
 // chunks, a stack of chunk chaining values merged as chunks complete, the
 // root node recompressed with increasing counters for extended output).
+#include <immintrin.h>
 #include <sys/mman.h>
 
 #include <algorithm>
@@ -72,6 +73,139 @@ void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, u
   }
 }
 
+// Eight full chunks at once, one per 32-bit lane of AVX2 vectors: chunk i
+// at p + 1024 i with chunk counter ctr + i; its chaining value goes to
+// cvs[8 i .. 8 i + 8).  The message words of a block are gathered across
+// the chunks; rotations by 16 and 8 are byte shuffles.
+__attribute__((target("avx2"))) void hash8_chunks_avx2(const uint8_t* p, uint64_t ctr,
+                                                      uint32_t* cvs) {
+  const __m256i rot16 = _mm256_setr_epi8(2, 3, 0, 1, 6, 7, 4, 5, 10, 11, 8, 9, 14, 15, 12, 13,
+                                         2, 3, 0, 1, 6, 7, 4, 5, 10, 11, 8, 9, 14, 15, 12, 13);
+  const __m256i rot8 = _mm256_setr_epi8(1, 2, 3, 0, 5, 6, 7, 4, 9, 10, 11, 8, 13, 14, 15, 12,
+                                        1, 2, 3, 0, 5, 6, 7, 4, 9, 10, 11, 8, 13, 14, 15, 12);
+  const __m256i idx = _mm256_setr_epi32(0, 256, 512, 768, 1024, 1280, 1536, 1792);
+  alignas(32) uint32_t lo[8], hi[8];
+  for (int i = 0; i < 8; i++) {
+    lo[i] = (uint32_t)(ctr + i);
+    hi[i] = (uint32_t)((ctr + i) >> 32);
+  }
+  const __m256i ctr_lo = _mm256_load_si256((const __m256i*)lo);
+  const __m256i ctr_hi = _mm256_load_si256((const __m256i*)hi);
+  __m256i h[8];
+  for (int w = 0; w < 8; w++) h[w] = _mm256_set1_epi32((int)kIV[w]);
+#define ZP_ROTR12(x) _mm256_or_si256(_mm256_srli_epi32(x, 12), _mm256_slli_epi32(x, 20))
+#define ZP_ROTR7(x) _mm256_or_si256(_mm256_srli_epi32(x, 7), _mm256_slli_epi32(x, 25))
+#define ZP_G8(a, bb, c, d, x, y)                                                  \
+  v[a] = _mm256_add_epi32(_mm256_add_epi32(v[a], v[bb]), x);                     \
+  v[d] = _mm256_shuffle_epi8(_mm256_xor_si256(v[d], v[a]), rot16);               \
+  v[c] = _mm256_add_epi32(v[c], v[d]);                                           \
+  v[bb] = ZP_ROTR12(_mm256_xor_si256(v[bb], v[c]));                              \
+  v[a] = _mm256_add_epi32(_mm256_add_epi32(v[a], v[bb]), y);                     \
+  v[d] = _mm256_shuffle_epi8(_mm256_xor_si256(v[d], v[a]), rot8);                \
+  v[c] = _mm256_add_epi32(v[c], v[d]);                                           \
+  v[bb] = ZP_ROTR7(_mm256_xor_si256(v[bb], v[c]));
+  for (int b = 0; b < 16; b++) {
+    __m256i m[16];
+    for (int w = 0; w < 16; w++)
+      m[w] = _mm256_i32gather_epi32((const int*)(p + 64 * b + 4 * w), idx, 4);
+    __m256i v[16];
+    for (int w = 0; w < 8; w++) v[w] = h[w];
+    for (int w = 0; w < 4; w++) v[8 + w] = _mm256_set1_epi32((int)kIV[w]);
+    v[12] = ctr_lo;
+    v[13] = ctr_hi;
+    v[14] = _mm256_set1_epi32(64);
+    v[15] = _mm256_set1_epi32((int)((b == 0 ? kChunkStart : 0u) | (b == 15 ? kChunkEnd : 0u)));
+    for (int r = 0; r < 7; r++) {
+      ZP_G8(0, 4, 8, 12, m[0], m[1])
+      ZP_G8(1, 5, 9, 13, m[2], m[3])
+      ZP_G8(2, 6, 10, 14, m[4], m[5])
+      ZP_G8(3, 7, 11, 15, m[6], m[7])
+      ZP_G8(0, 5, 10, 15, m[8], m[9])
+      ZP_G8(1, 6, 11, 12, m[10], m[11])
+      ZP_G8(2, 7, 8, 13, m[12], m[13])
+      ZP_G8(3, 4, 9, 14, m[14], m[15])
+      __m256i t[16];
+      for (int i = 0; i < 16; i++) t[i] = m[kSchedule[i]];
+      for (int i = 0; i < 16; i++) m[i] = t[i];
+    }
+    for (int w = 0; w < 8; w++) h[w] = _mm256_xor_si256(v[w], v[w + 8]);
+  }
+  alignas(32) uint32_t tmp[8][8];
+  for (int w = 0; w < 8; w++) _mm256_store_si256((__m256i*)tmp[w], h[w]);
+  for (int i = 0; i < 8; i++)
+    for (int w = 0; w < 8; w++) cvs[8 * i + w] = tmp[w][i];
+#undef ZP_G8
+#undef ZP_ROTR7
+#undef ZP_ROTR12
+}
+
+// Sixteen full chunks at once with AVX-512 (native 32-bit rotates); the
+// same layout as hash8_chunks_avx2.
+__attribute__((target("avx512f"))) void hash16_chunks_avx512(const uint8_t* p, uint64_t ctr,
+                                                            uint32_t* cvs) {
+  const __m512i idx = _mm512_setr_epi32(0, 256, 512, 768, 1024, 1280, 1536, 1792, 2048, 2304,
+                                        2560, 2816, 3072, 3328, 3584, 3840);
+  alignas(64) uint32_t lo[16], hi[16];
+  for (int i = 0; i < 16; i++) {
+    lo[i] = (uint32_t)(ctr + i);
+    hi[i] = (uint32_t)((ctr + i) >> 32);
+  }
+  const __m512i ctr_lo = _mm512_load_si512(lo);
+  const __m512i ctr_hi = _mm512_load_si512(hi);
+  __m512i h[8];
+  for (int w = 0; w < 8; w++) h[w] = _mm512_set1_epi32((int)kIV[w]);
+#define ZP_G16(a, bb, c, d, x, y)                                               \
+  v[a] = _mm512_add_epi32(_mm512_add_epi32(v[a], v[bb]), x);                   \
+  v[d] = _mm512_ror_epi32(_mm512_xor_si512(v[d], v[a]), 16);                   \
+  v[c] = _mm512_add_epi32(v[c], v[d]);                                         \
+  v[bb] = _mm512_ror_epi32(_mm512_xor_si512(v[bb], v[c]), 12);                 \
+  v[a] = _mm512_add_epi32(_mm512_add_epi32(v[a], v[bb]), y);                   \
+  v[d] = _mm512_ror_epi32(_mm512_xor_si512(v[d], v[a]), 8);                    \
+  v[c] = _mm512_add_epi32(v[c], v[d]);                                         \
+  v[bb] = _mm512_ror_epi32(_mm512_xor_si512(v[bb], v[c]), 7);
+  for (int b = 0; b < 16; b++) {
+    __m512i m[16];
+    for (int w = 0; w < 16; w++)
+      m[w] = _mm512_i32gather_epi32(idx, (const void*)(p + 64 * b + 4 * w), 4);
+    __m512i v[16];
+    for (int w = 0; w < 8; w++) v[w] = h[w];
+    for (int w = 0; w < 4; w++) v[8 + w] = _mm512_set1_epi32((int)kIV[w]);
+    v[12] = ctr_lo;
+    v[13] = ctr_hi;
+    v[14] = _mm512_set1_epi32(64);
+    v[15] = _mm512_set1_epi32((int)((b == 0 ? kChunkStart : 0u) | (b == 15 ? kChunkEnd : 0u)));
+    for (int r = 0; r < 7; r++) {
+      ZP_G16(0, 4, 8, 12, m[0], m[1])
+      ZP_G16(1, 5, 9, 13, m[2], m[3])
+      ZP_G16(2, 6, 10, 14, m[4], m[5])
+      ZP_G16(3, 7, 11, 15, m[6], m[7])
+      ZP_G16(0, 5, 10, 15, m[8], m[9])
+      ZP_G16(1, 6, 11, 12, m[10], m[11])
+      ZP_G16(2, 7, 8, 13, m[12], m[13])
+      ZP_G16(3, 4, 9, 14, m[14], m[15])
+      __m512i t[16];
+      for (int i = 0; i < 16; i++) t[i] = m[kSchedule[i]];
+      for (int i = 0; i < 16; i++) m[i] = t[i];
+    }
+    for (int w = 0; w < 8; w++) h[w] = _mm512_xor_si512(v[w], v[w + 8]);
+  }
+  alignas(64) uint32_t tmp[8][16];
+  for (int w = 0; w < 8; w++) _mm512_store_si512(tmp[w], h[w]);
+  for (int i = 0; i < 16; i++)
+    for (int w = 0; w < 8; w++) cvs[8 * i + w] = tmp[w][i];
+#undef ZP_G16
+}
+
+bool host_has_avx2() {
+  static const bool yes = __builtin_cpu_supports("avx2");
+  return yes;
+}
+
+bool host_has_avx512() {
+  static const bool yes = __builtin_cpu_supports("avx512f");
+  return yes;
+}
+
 void words_le(const uint8_t* b, uint32_t* w, int n) {
   for (int i = 0; i < n; i++)
     w[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) |
@@ -115,6 +249,22 @@ class Blake3 {
         chunk_node().chaining(cv);
         push_chunk(cv, chunk_ + 1);
         reset_chunk(chunk_ + 1);
+      }
+      // At a chunk boundary with more than sixteen (eight) chunks of input
+      // left, so none of them is the last chunk: sixteen (eight) at once.
+      while (chunk_len() == 0 && n > 16 * 1024 && host_has_avx512()) {
+        uint32_t cvs[128];
+        hash16_chunks_avx512(p, chunk_, cvs);
+        for (int i = 0; i < 16; i++) push_chunk_cv(cvs + 8 * i);
+        p += 16 * 1024;
+        n -= 16 * 1024;
+      }
+      while (chunk_len() == 0 && n > 8 * 1024 && host_has_avx2()) {
+        uint32_t cvs[64];
+        hash8_chunks_avx2(p, chunk_, cvs);
+        for (int i = 0; i < 8; i++) push_chunk_cv(cvs + 8 * i);
+        p += 8 * 1024;
+        n -= 8 * 1024;
       }
       if (buf_len_ == 64) {  // a full block is compressed only once more input follows
         uint32_t m[16], o[16];
@@ -406,10 +556,35 @@ int zp_transcript_absorb_slice(zp_transcript* t, const uint8_t* bytes, uint64_t 
 int zp_transcript_absorb_entries(zp_transcript* t, const uint8_t* bytes, uint64_t entry_bytes,
                                  uint64_t count) {
   if (!t || (count && entry_bytes && !bytes)) return fail(ZP_ERR_INVALID_PARAM, "null argument");
-  for (uint64_t i = 0; i < count; i++) {
-    t->absorb_tag(0x6);
-    t->absorb(bytes + i * entry_bytes, entry_bytes);
-    t->absorb_tag(0x7);
+  // The framed entries (0x6 || entry || 0x7) are assembled in a buffer and
+  // absorbed in large pieces, so the hasher can take many chunks at once.
+  constexpr size_t kBuf = 256u << 10;
+  const uint64_t F = entry_bytes + 2;
+  if (F > kBuf) {
+    for (uint64_t i = 0; i < count; i++) {
+      t->absorb_tag(0x6);
+      t->absorb(bytes + i * entry_bytes, entry_bytes);
+      t->absorb_tag(0x7);
+    }
+    return ZP_OK;
+  }
+  std::vector<uint8_t> buf;
+  try {
+    buf.resize(kBuf);
+  } catch (const std::bad_alloc&) {
+    return fail(ZP_ERR_OOM, "host allocation failed");
+  }
+  const uint64_t per = kBuf / F;
+  for (uint64_t i0 = 0; i0 < count; i0 += per) {
+    const uint64_t k = std::min(per, count - i0);
+    uint8_t* q = buf.data();
+    for (uint64_t i = 0; i < k; i++) {
+      *q++ = 0x6;
+      if (entry_bytes) std::memcpy(q, bytes + (i0 + i) * entry_bytes, entry_bytes);
+      q += entry_bytes;
+      *q++ = 0x7;
+    }
+    t->absorb(buf.data(), (size_t)(q - buf.data()));
   }
   return ZP_OK;
 }
