@@ -71,16 +71,20 @@ static_assert((kMagicBits & 0xffffffffull) == 0, "bias must be a multiple of 2^3
 // k = 16w .. 16w+15 in "spread" order: bit i = 4v + b of the group sits at
 // position 8b + v, so (word >> v) & 0x01010101 is operand register v of the
 // MFMA B fragment (bytes b = 0..3 for k = 16w + 4v + b) in two instructions.
-// One thread per (row, c, w, 4 planes), planes fastest: a lane writes its 4
-// words as one 16-byte store (the 8 lanes of a (c, w) cover 128 contiguous
-// bytes for D = 32), and the chunks whose octet reversals are adjacent
-// follow, so one load instruction reads adjacent evals.
+// One thread per (row, c, w, 16 planes; D is a multiple of 16 on this
+// path), planes fastest: a lane writes its 16 words as four 16-byte stores
+// (the 2 lanes of a (c, w) cover 128 contiguous bytes for D = 32), and the
+// chunks whose octet reversals are adjacent follow, so one load instruction
+// reads adjacent evals.  With 4 planes per lane, eight lanes loaded the same
+// 16 evals and the pre-pass was bound by load issue (12 -> ~8 µs at 2^20).
+constexpr uint32_t kPlanesPerLane = 16;
 template <int NC>
 __global__ void __launch_bounds__(256)
 bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ planes,
                 uint32_t row_len, uint32_t bl, uint32_t W, uint32_t D, uint32_t depth,
                 uint64_t rows) {
-  const uint32_t PQ = D / 4;
+  constexpr uint32_t PPL = kPlanesPerLane;
+  const uint32_t PQ = D / PPL;
   const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= rows * NC * W * PQ) return;
   const uint32_t pq = (uint32_t)(idx % PQ);
@@ -99,16 +103,21 @@ bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ plane
   uint64_t xs[16];
 #pragma unroll
   for (int i = 0; i < 16; i++) xs[i] = x[(uint64_t)NC * min(16 * w + i, bl - 1)];
-  uint32_t v[4] = {0, 0, 0, 0};
+  uint32_t v[PPL];
+#pragma unroll
+  for (int q = 0; q < (int)PPL; q++) v[q] = 0;
 #pragma unroll
   for (int i = 0; i < 16; i++) {
     const uint32_t k = 16 * w + i;
-    const uint32_t bits = k < bl ? (uint32_t)(xs[i] >> (4 * pq)) : 0u;
+    const uint32_t bits = k < bl ? (uint32_t)(xs[i] >> (PPL * pq)) : 0u;
     const uint32_t pos = 8 * (i & 3) + (i >> 2);
 #pragma unroll
-    for (int q = 0; q < 4; q++) v[q] |= ((bits >> q) & 1u) << pos;
+    for (int q = 0; q < (int)PPL; q++) v[q] |= ((bits >> q) & 1u) << pos;
   }
-  *(uint4*)(planes + ((row * NC + c) * W + w) * D + 4 * pq) = make_uint4(v[0], v[1], v[2], v[3]);
+  uint4* dst = (uint4*)(planes + ((row * NC + c) * W + w) * D + PPL * pq);
+#pragma unroll
+  for (int q = 0; q < (int)PPL / 4; q++)
+    dst[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
 }
 
 // Radix-8 butterflies by symmetry.  Output t of a butterfly is
@@ -407,7 +416,7 @@ int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t 
   const uint64_t bytes = rows * NC * W * D * sizeof(uint32_t);
   uint32_t* planes = nullptr;
   ZP_TRY(dev_alloc((void**)&planes, bytes, s));
-  const uint64_t threads = rows * NC * W * (D / 4);
+  const uint64_t threads = rows * NC * W * (D / kPlanesPerLane);
   hipLaunchKernelGGL((bitplane_kernel<NC>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,
                      s, evals, planes, (uint32_t)cd->row_len, cd->bl, W, D, cd->depth, rows);
   int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "bitplane launch");
