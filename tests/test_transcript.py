@@ -69,6 +69,22 @@ def test_multi_chunk_absorbs(zgh, oracle, sizes):
     assert t.get_challenge(64) == o.challenge(64)
 
 
+@pytest.mark.parametrize("entry,count", [(300000, 2), (262142, 3), (262143, 2), (1, 70000)])
+def test_absorb_entries_bulk_and_large(zgh, oracle, entry, count):
+    # Entries are framed into 256 KiB pieces; a framed entry larger than a
+    # piece (entry + 2 tag bytes > 262144) is absorbed entry by entry.
+    import numpy as np
+    data = np.random.default_rng(entry + count).integers(0, 256, entry * count,
+                                                          dtype=np.uint8).tobytes()
+    t, o = zgh.PcsProverTranscript(), oracle.Transcript()
+    t.absorb_slice(b"x")
+    o.absorb_slice(b"x")
+    t.absorb_entries(data, entry)
+    for i in range(count):
+        o.absorb_slice(data[entry * i:entry * (i + 1)])
+    assert t.get_challenge(32) == o.challenge(32)
+
+
 def test_random_field_cfg_matches(zgh, oracle):
     for seed in range(4):
         root = bytes((seed * 37 + i) % 256 for i in range(32))
