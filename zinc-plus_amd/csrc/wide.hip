@@ -189,7 +189,7 @@ wide_base_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
 // v_mfma_i32_16x16x64_i8 per 64 k (|sum| < 2^22), and
 //   out = sum_{a,b} 2^(7a + 8b) (F_a D_b)   modulo 2^(64L).
 // A one-limb input (|x| < 2^63, sign-extended) needs only 9 digits (8 for
-// L = 1); a full input needs 8L.
+// L = 1), an i32 input 5; a full input needs 8L.
 typedef int wv4i __attribute__((ext_vector_type(4)));
 
 inline uint32_t wide_digit_count(const WideLayout& lay, int L) {
