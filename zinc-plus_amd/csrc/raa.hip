@@ -268,15 +268,19 @@ int launch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t
 // shift and one bit count (which adds the checkpoint).  Plane rows are an
 // odd number of 32-bit words apart (bits) and of bank words apart
 // (checkpoints), so the 64 lanes of a lookup (one per plane, same position)
-// hit 64 different banks.  A workgroup holds all planes of one row (64
-// planes, N = 8192: 64 KiB of bits + 32 KiB of checkpoints).  Phase 1 puts
-// positions in lanes and turns each plane's bits into 64-bit words with one
-// ballot per plane; phase 2 puts planes in lanes, so each output position
-// (4*D bytes) is written by one store instruction.  perm2 is copied to LDS
-// before any store: the memory counter also counts stores, so a global load
-// of perm2 between the stores would wait for all of them.
+// hit different banks.  A workgroup holds 32 planes of one row (N = 8192:
+// 32 KiB of bits + 16 KiB of checkpoints + 16 KiB of perm2 as 16-bit
+// indices), so two workgroups share a CU and one's stores overlap the
+// other's scans; at D = 64 the two halves of a row are separate
+// workgroups.  Phase 1 puts positions in lanes and turns each plane's bits
+// into 64-bit words with one ballot per plane; phase 2 puts planes in
+// lanes, two positions per wave, so each store instruction writes two
+// 128-byte runs.  perm2 is copied to LDS before any store: the memory
+// counter also counts stores, so a global load of perm2 between the stores
+// would wait for all of them.
 constexpr uint32_t kRowThreads = 512;
 constexpr uint32_t kRowWaves = kRowThreads / 64;
+constexpr uint32_t kRowPlanes = 32;  // planes per workgroup
 
 // Word strides of one plane: bits (odd, in 32-bit words) and checkpoints
 // (an odd number of 32-bit bank words, i.e. 2 mod 4 in 16-bit entries).
@@ -294,28 +298,33 @@ __device__ __forceinline__ uint32_t rank_word(uint32_t word, uint32_t m, uint32_
   return __popc(t) + base;
 }
 
+// D planes per position in the output row, PG of them (planes PG*pg ..
+// PG*pg + PG - 1) per workgroup: blockIdx.x = row * (D / PG) + pg.
 template <int D>
 __global__ void __launch_bounds__(kRowThreads)
 raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
                const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
                uint32_t row_len, uint32_t N, uint64_t row0) {
-  constexpr uint32_t SUB = 64 / D;           // position streams per wave
+  constexpr uint32_t PG = kRowPlanes;
+  constexpr uint32_t SUB = 64 / PG;          // position streams per wave
   constexpr uint32_t NS = kRowWaves * SUB;   // position streams per workgroup
+  constexpr uint32_t NPG = D / PG;
   extern __shared__ uint32_t lds32[];
   const uint32_t W = N / 64;                 // 64-bit words (ballots) per plane
   const uint32_t W32 = N / 32;               // 32-bit words per plane
   const uint32_t WP = raa_bits_stride(N);
   const uint32_t CP = raa_cp_stride(N);
-  uint32_t* bits = lds32;                                        // [D][WP]
-  uint16_t* cp = (uint16_t*)(lds32 + (size_t)D * WP);            // [D][CP]
-  uint32_t* part = lds32 + (size_t)D * WP + ((size_t)D * CP) / 2;
-  uint32_t* p2 = part + 2 * kRowThreads;                         // [N] perm2
+  uint32_t* bits = lds32;                                        // [PG][WP]
+  uint16_t* cp = (uint16_t*)(lds32 + (size_t)PG * WP);           // [PG][CP]
+  uint32_t* part = lds32 + (size_t)PG * WP + ((size_t)PG * CP) / 2;
+  uint16_t* p2 = (uint16_t*)(part + 2 * kRowThreads);            // [N] perm2 (N <= 2^15)
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint64_t row = row0 + blockIdx.x;
+  const uint64_t row = row0 + blockIdx.x / NPG;
+  const uint32_t poff = PG * (blockIdx.x % NPG);
   const uint64_t* x = evals + row * row_len;
   const uint32_t rmask = row_len - 1;
 
-  for (uint32_t i = tid; i < N; i += kRowThreads) p2[i] = perm2[i];
+  for (uint32_t i = tid; i < N; i += kRowThreads) p2[i] = (uint16_t)perm2[i];
   // Phase 1a: y1 bit-vectors, 64 positions per wave step; the permutation
   // and gather loads of up to kPre steps are issued together.
   constexpr uint32_t kPre = 4;
@@ -333,18 +342,19 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
     for (uint32_t k = 0; k < kPre; k++) {
       const uint32_t blk = b0 + k * kRowWaves;
       if (blk >= W) break;
-      // Lane q collects plane q's word from the q-th ballot.
-      const uint32_t lo = (uint32_t)xv[k], hi = (uint32_t)(xv[k] >> 32);
+      // Lane q collects plane poff + q's word from the q-th ballot.
+      const uint64_t xq = xv[k] >> poff;
+      const uint32_t lo = (uint32_t)xq, hi = (uint32_t)(xq >> 32);
       uint32_t wlo = 0, whi = 0;
 #pragma unroll
-      for (int q = 0; q < D; q++) {
+      for (int q = 0; q < PG; q++) {
         const uint64_t m = __ballot((((q < 32) ? (lo >> q) : (hi >> (q - 32))) & 1u) != 0u);
         if (lane == (uint32_t)q) {
           wlo = (uint32_t)m;
           whi = (uint32_t)(m >> 32);
         }
       }
-      if (lane < (uint32_t)D) {
+      if (lane < (uint32_t)PG) {
         bits[lane * WP + 2 * blk] = wlo;
         bits[lane * WP + 2 * blk + 1] = whi;
       }
@@ -355,8 +365,8 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   // Phase 1b: checkpoints (exclusive counts per word), one thread per
   // (plane, segment of words) plus a scan over the segments.
   {
-    constexpr uint32_t SEGS = kRowThreads / D;
-    const uint32_t p = tid % D, seg = tid / D;
+    constexpr uint32_t SEGS = kRowThreads / PG;
+    const uint32_t p = tid % PG, seg = tid / PG;
     const uint32_t ws = (W32 + SEGS - 1) / SEGS;
     const uint32_t w0 = min(W32, seg * ws), w1 = min(W32, w0 + ws);
     uint32_t cnt = 0;
@@ -364,7 +374,7 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
     part[tid] = cnt;
     __syncthreads();
     uint32_t before = 0;
-    for (uint32_t sg = 0; sg < seg; sg++) before += part[sg * D + p];
+    for (uint32_t sg = 0; sg < seg; sg++) before += part[sg * PG + p];
     for (uint32_t w = w0; w < w1; w++) {
       cp[p * CP + w] = (uint16_t)before;
       before += __popc(bits[p * WP + w]);
@@ -376,7 +386,7 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   // steps.  In each step stream s takes K consecutive positions, prefixes
   // them in registers, and the streams exchange their totals through LDS
   // (double-buffered, one barrier per step) to get their carries.
-  const uint32_t sub = lane / D, p = lane % D;
+  const uint32_t sub = lane / PG, p = lane % PG;
   const uint32_t s = wave * SUB + sub;
   constexpr uint32_t K = 64 / SUB;
   const uint32_t steps = N / (NS * K);
@@ -386,7 +396,7 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
     const uint32_t w = m >> 5;
     return rank_word(bp[w], m, cpp[w]);
   };
-  int32_t* orow = out + row * N * D + p;
+  int32_t* orow = out + row * N * D + poff + p;
   uint32_t run = 0;  // every position of the earlier steps
   for (uint32_t st = 0; st < steps; st++) {
     const uint32_t base = (st * NS + s) * K;
@@ -402,13 +412,13 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
     }
 #pragma unroll
     for (uint32_t k = 1; k < K; k++) v[k] += v[k - 1];
-    uint32_t* tot = part + (st & 1) * NS * D;
-    tot[s * D + p] = v[K - 1];
+    uint32_t* tot = part + (st & 1) * NS * PG;
+    tot[s * PG + p] = v[K - 1];
     __syncthreads();
     uint32_t before = 0, all = 0;
 #pragma unroll
     for (uint32_t q = 0; q < NS; q++) {
-      const uint32_t t = tot[q * D + p];
+      const uint32_t t = tot[q * PG + p];
       all += t;
       before += q < s ? t : 0u;
     }
@@ -423,21 +433,23 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
 // buffers of per-stream totals: 2 * 8 * 64 words), and perm2.
 constexpr uint32_t kRowPartWords = 2 * kRowThreads;
 
-size_t raa_row_lds(uint32_t D, uint32_t N) {
-  return (size_t)D * raa_bits_stride(N) * 4 + (size_t)D * raa_cp_stride(N) * 2 +
-         (size_t)kRowPartWords * 4 + (size_t)N * 4;
+size_t raa_row_lds(uint32_t N) {
+  return (size_t)kRowPlanes * raa_bits_stride(N) * 4 +
+         (size_t)kRowPlanes * raa_cp_stride(N) * 2 + (size_t)kRowPartWords * 4 + (size_t)N * 2;
 }
 
 template <int D>
 int launch_raa_row(const zp_code* cd, const uint64_t* evals, uint64_t rows, int32_t* out,
                    hipStream_t s) {
   const uint32_t N = (uint32_t)cd->N;
-  const size_t lds = raa_row_lds(D, N);
+  const size_t lds = raa_row_lds(N);
   ZP_TRY(grant_lds<raa_row_kernel<D>>(lds));
-  for (uint64_t r0 = 0; r0 < rows; r0 += 0x7fffffffull) {
-    const uint32_t nr = (uint32_t)std::min<uint64_t>(0x7fffffffull, rows - r0);
-    hipLaunchKernelGGL((raa_row_kernel<D>), dim3(nr), dim3(kRowThreads), lds, s, evals, out,
-                       cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, r0);
+  constexpr uint32_t NPG = D / kRowPlanes;
+  const uint64_t max_rows = 0x7fffffffull / NPG;
+  for (uint64_t r0 = 0; r0 < rows; r0 += max_rows) {
+    const uint32_t nr = (uint32_t)std::min<uint64_t>(max_rows, rows - r0);
+    hipLaunchKernelGGL((raa_row_kernel<D>), dim3(nr * NPG), dim3(kRowThreads), lds, s, evals,
+                       out, cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, r0);
     ZP_HIP(hipGetLastError());
   }
   return ZP_OK;
@@ -467,7 +479,7 @@ bool raa_full_row_eligible(const zp_code* cd, uint32_t D) {
   // Plain code, 32 or 64 planes, every stream a whole number of positions,
   // and the bit-vectors within LDS.
   return cd->kind == ZP_CODE_RAA && (D == 64 || D == 32) && cd->N >= 1024 &&
-         raa_row_lds(D, (uint32_t)cd->N) <= 150 * 1024;
+         cd->N <= (1u << 15) && raa_row_lds((uint32_t)cd->N) <= 150 * 1024;
 }
 
 // variant: 0 full-row kernel where eligible, 1 plane groups of 8 (at
