@@ -135,8 +135,11 @@ def leg_raa(torch, zipgpu, synth, dev):
     pp = zp.setup(n, code)
     out = torch.empty((pp.num_rows, code.codeword_len(), d), dtype=torch.int32, device=dev)
     stream = torch.cuda.Stream()
+    # Back-to-back launches from idle run fast for a few launches, slow down
+    # for a few more and then settle (the board's power management), so the
+    # average spans 20 launches after 5.
     enc = event_ms(torch, lambda s: code.encode_rows(x, pp.num_rows, types, out=out, stream=s),
-                   stream, 5)
+                   stream, 20, warmup=5)
     del out
 
     def step():

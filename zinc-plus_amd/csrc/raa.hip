@@ -267,8 +267,8 @@ int launch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t
 // The shift keeps bits 0 .. m % 32 of the word, so a rank is one xor, one
 // shift and one bit count (which adds the checkpoint).  Plane rows are an
 // odd number of 32-bit words apart (bits) and of bank words apart
-// (checkpoints), so the 64 lanes of a lookup (one per plane, same position)
-// hit different banks.  A workgroup holds 32 planes of one row (N = 8192:
+// (checkpoints), so the 32 lanes that look up one position (one lane per
+// plane) hit different banks.  A workgroup holds 32 planes of one row (N = 8192:
 // 32 KiB of bits + 16 KiB of checkpoints + 16 KiB of perm2 as 16-bit
 // indices), so two workgroups share a CU and one's stores overlap the
 // other's scans; at D = 64 the two halves of a row are separate
