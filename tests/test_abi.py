@@ -55,3 +55,20 @@ def test_setup_validation():
         zipgpu.ZipPlus.setup(3 << 10, Code())
     with pytest.raises(zipgpu.ShapeError):
         zipgpu.ZipPlus.setup(128, Code())
+
+
+def test_null_buffers_rejected_before_device_work():
+    # Argument checks run before any HIP call, so they hold without a GPU.
+    from zipgpu import _lib
+    lib = _lib.load()
+    root = (ctypes.c_uint8 * 32)()
+    h = ctypes.c_void_p()
+    assert lib.zp_merkle_root(None, 4, 8, 32, root, None) == _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_merkle_root(ctypes.c_void_p(64), 4, 8, 32, None, None) == \
+        _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_merkle_root(ctypes.c_void_p(64), 4, 8, 0, root, None) == _lib.ZP_ERR_SHAPE
+    assert lib.zp_commit(ctypes.c_void_p(64), 0, 32, None, 4, 1, ctypes.byref(h), root,
+                         None) == _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_combined_row(32, 5, 4, 16, 1, None, None, None, None, None) == \
+        _lib.ZP_ERR_INVALID_PARAM
+    assert b"null" in lib.zp_last_error()

@@ -385,7 +385,8 @@ int zp_encode_wide(const zp_code* code, uint32_t L, const uint64_t* d_in, uint64
 int zp_merkle_root(const void* d_rows, uint64_t num_rows, uint64_t width, uint32_t elem,
                    uint8_t h_root[32], zp_stream stream) {
   hipStream_t s = (hipStream_t)stream;
-  if (num_rows == 0 || width == 0) return fail(ZP_ERR_SHAPE, "empty matrix");
+  if (!d_rows || !h_root) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  if (num_rows == 0 || width == 0 || elem == 0) return fail(ZP_ERR_SHAPE, "empty matrix");
   if (!pow2(width)) return fail(ZP_ERR_SHAPE, "row width must be a power of two");
   uint8_t* layers = nullptr;
   ZP_TRY(dev_alloc((void**)&layers, (2 * width - 1) * 32, s));
@@ -410,7 +411,8 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_eval
               uint64_t num_rows, uint32_t batch, zp_hint** hint_out, uint8_t h_root[32],
               zp_stream stream) {
   hipStream_t s = (hipStream_t)stream;
-  if (!code || !hint_out) return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  if (!code || !hint_out || !d_evals || !h_root)
+    return fail(ZP_ERR_INVALID_PARAM, "null argument");
   *hint_out = nullptr;
   zp_hint* h = nullptr;
   ZP_TRY(hint_alloc(code, eval_type, D, num_rows, batch, s, &h));
@@ -445,6 +447,7 @@ const void* zp_hint_codewords(const zp_hint* h) { return h ? h->d_cw : nullptr; 
 
 int zp_hint_copy_codewords(const zp_hint* h, uint32_t poly, void* h_out, zp_stream stream) {
   if (!h || poly >= h->batch) return fail(ZP_ERR_INVALID_PARAM, "bad hint or poly index");
+  if (!h_out) return fail(ZP_ERR_INVALID_PARAM, "null output");
   hipStream_t s = (hipStream_t)stream;
   ZP_HIP(hipMemcpyAsync(h_out, h->d_cw + poly * h->per_poly, h->per_poly, hipMemcpyDeviceToHost,
                         s));
@@ -454,6 +457,7 @@ int zp_hint_copy_codewords(const zp_hint* h, uint32_t poly, void* h_out, zp_stre
 
 int zp_hint_copy_layer(const zp_hint* h, uint32_t level, uint8_t* h_out, zp_stream stream) {
   if (!h || level >= h->height) return fail(ZP_ERR_INVALID_PARAM, "bad hint or layer");
+  if (!h_out) return fail(ZP_ERR_INVALID_PARAM, "null output");
   uint64_t off = 0, cnt = h->N;
   for (uint32_t l = 0; l < level; l++) {
     off += cnt;
@@ -468,6 +472,7 @@ int zp_hint_copy_layer(const zp_hint* h, uint32_t level, uint8_t* h_out, zp_stre
 int zp_merkle_prove(const zp_hint* h, uint64_t leaf, uint8_t* h_siblings, uint32_t* h_len,
                     zp_stream stream) {
   if (!h || leaf >= h->N) return fail(ZP_ERR_INVALID_PARAM, "Invalid leaf index");
+  if (!h_siblings || !h_len) return fail(ZP_ERR_INVALID_PARAM, "null output");
   hipStream_t s = (hipStream_t)stream;
   uint64_t off = 0, cnt = h->N, idx = leaf;
   uint32_t k = 0;
@@ -486,6 +491,7 @@ int zp_merkle_prove(const zp_hint* h, uint64_t leaf, uint8_t* h_siblings, uint32
 
 int zp_hint_open_column(const zp_hint* h, uint64_t col, void* h_out, zp_stream stream) {
   if (!h || col >= h->N) return fail(ZP_ERR_INVALID_PARAM, "bad hint or column");
+  if (!h_out) return fail(ZP_ERR_INVALID_PARAM, "null output");
   hipStream_t s = (hipStream_t)stream;
   const uint64_t rows = h->num_rows * h->batch;
   ZP_HIP(hipMemcpy2DAsync(h_out, h->elem, h->d_cw + col * h->elem, h->N * h->elem, h->elem, rows,
@@ -508,6 +514,8 @@ int zp_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
   if (D == 0 || D > 64) return fail(ZP_ERR_INVALID_PARAM, "degree bound must be within 1..64");
   if (L < 3 || L > 8) return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 3..8");
   if (batch == 0 || num_rows == 0 || row_len == 0) return fail(ZP_ERR_SHAPE, "empty input");
+  if (!d_evals || !d_alphas || !d_coeffs || !d_out)
+    return fail(ZP_ERR_INVALID_PARAM, "null buffer");
   void* scratch = nullptr;
   ZP_TRY(dev_alloc(&scratch, combined_row_scratch_bytes(D, row_len, batch, num_rows), s));
   int rc = launch_combined_row(D, L, num_rows, row_len, batch, d_evals, d_alphas, d_coeffs, d_out,
