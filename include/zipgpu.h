@@ -81,6 +81,10 @@ const char* zp_last_error(void);
  *                   (default), 1 = the per-element kernels.
  *   "merkle_tree":  0 = Merkle layers with one 4-lane quad per parent
  *                   (default), 1 = one thread per parent.
+ *   "chunk_tree":   0 = a column's chunk tree (4 to 2048 chunks, a power
+ *                   of two) folded in registers by up to 256 lanes and
+ *                   merged by 4-lane quads (default), 1 = the whole tree
+ *                   in LDS, or one launch per level, for every chunk count.
  * Returns ZP_ERR_INVALID_PARAM for an unknown name or value. */
 int zp_set_option(const char* name, int64_t value);
 

@@ -190,6 +190,53 @@ def test_merkle_root(zg, dev, oracle, rows, width, elem):
     assert got == oracle.merkle_root(m, rows, width, elem)
 
 
+# Columns of 4 to 2048 chunks (the register-fold chunk tree with 1, 2, 4 or
+# 8 chunks per lane and 4 to 256 lanes per column; one with a partial last
+# chunk, one with 200-byte elements, one single column) and of 2, 5, 6 and
+# 4096 chunks (the LDS tree kernel or the per-level kernel in both
+# settings).
+@pytest.mark.parametrize("rows,width,elem", [
+    (16, 64, 256), (32, 64, 256), (64, 128, 256), (128, 32, 256), (15, 16, 256), (20, 8, 200),
+    (64, 1, 256), (256, 8, 256), (2048, 8, 256), (4096, 4, 256), (8192, 2, 256),
+    (4, 8192, 1024), (72, 4, 64), (5, 64, 256), (12, 8, 512), (16384, 2, 256)])
+def test_chunk_tree_kernels_agree(zg, dev, oracle, rows, width, elem):
+    say("chunk trees", rows, width, elem)
+    m = synth.splitmix64(rows * 7 + width + elem, rows * width * elem // 8 + 1).view(np.uint8)
+    m = m[:rows * width * elem].reshape(rows, width, elem)
+    md = to_dev(m, dev)
+    roots = {}
+    try:
+        for opt in (1, -1):
+            zg.set_option("chunk_tree", opt)
+            roots[opt] = zg.merkle_root(md, elem)
+    finally:
+        zg.set_option("chunk_tree", -1)
+    want = oracle.merkle_root(m, rows, width, elem)
+    assert roots[1] == want and roots[-1] == want
+
+
+@pytest.mark.parametrize("kind,n,row_len,D", [("iprs", 1 << 20, 1 << 14, 32),
+                                              ("raa", 1 << 22, 1 << 11, 64)])
+def test_chunk_tree_full_size_settings_agree(zg, dev, kind, n, row_len, D):
+    # The benchmarked commits: 16 chunks per column (4 per lane, 4 lanes) and
+    # 512 chunks per column (8 per lane, 64 lanes) against the LDS tree.
+    say("chunk trees full size", kind, n, D)
+    x = to_dev(synth.binary_polys(12, n, D), dev)
+    code = zg.IprsCode.new_with_optimal_depth(row_len) if kind == "iprs" else zg.RaaCode(row_len)
+    zp = zg.ZipPlus(zg.BinPolyTypes(D))
+    pp = zp.setup(n, code)
+    roots = {}
+    try:
+        for opt in (1, -1):
+            zg.set_option("chunk_tree", opt)
+            hint, comm = zp.commit(pp, [x])
+            roots[opt] = comm.root
+            hint.free()
+    finally:
+        zg.set_option("chunk_tree", -1)
+    assert roots[1] == roots[-1]
+
+
 # ---------------------------------------------------------------------------
 # Commit (encode + Merkle) and the hint
 # ---------------------------------------------------------------------------

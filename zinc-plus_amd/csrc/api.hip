@@ -23,7 +23,7 @@ namespace zpi {
 // Library-wide overrides of the kernel variants; -1 leaves the choice to
 // the code's tuned variant (zp_autotune) or the default.
 int g_force_iprs = -1, g_force_raa = -1, g_force_overlap = -1, g_force_wide = -1,
-    g_force_tree = -1;
+    g_force_tree = -1, g_force_ctree = -1;
 
 
 uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D) {
@@ -276,7 +276,8 @@ int zp_set_option(const char* name, int64_t value) {
                       {"raa_kernel", &g_force_raa, 2},
                       {"commit_overlap", &g_force_overlap, 1},
                       {"wide_base", &g_force_wide, 1},
-                      {"merkle_tree", &g_force_tree, 1}};
+                      {"merkle_tree", &g_force_tree, 1},
+                      {"chunk_tree", &g_force_ctree, 1}};
   for (const auto& o : opts) {
     if (std::strcmp(name, o.name) != 0) continue;
     if (value < -1 || value > o.max)

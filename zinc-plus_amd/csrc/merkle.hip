@@ -348,6 +348,74 @@ merkle_quad_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32
   }
 }
 
+// Chunk trees of columns of nchunks = C*M chunks (C in 4..256 and M in
+// 1..8, powers of two), C lanes per column and no chunk values in LDS.
+// Lane l folds chunks lM .. lM + M - 1, a perfect subtree, in registers:
+// chunk i is merged into the stack ctz(i + 1) times (the binary-counter
+// order, which for a power-of-two count is the level-by-level left
+// pairing).  The column's C/4 quads then merge its C nodes level by level
+// with parent_quad (two rounds at the first level, one after), reading each
+// pair of nodes as one 16-word message in LDS and writing the parent in
+// place after a barrier.  Columns past the last fold column 0 and store
+// nothing, so every quad stays whole for the DPP rotations.
+template <int M>
+__global__ void __launch_bounds__(256)
+cv_quad_tree_kernel(const uint32_t* __restrict__ cvs, uint64_t width, uint32_t C,
+                    uint8_t* __restrict__ leaves) {
+  static_assert(M >= 1 && M <= 8 && (M & (M - 1)) == 0, "M is 1, 2, 4 or 8");
+  constexpr int kDepth = M >= 8 ? 4 : M >= 4 ? 3 : M >= 2 ? 2 : 1;
+  __shared__ uint32_t nodes[256 * 8];
+  const uint32_t tid = threadIdx.x, q = tid & 3;
+  const uint32_t l = tid & (C - 1), col = tid / C;
+  const uint64_t j = (uint64_t)blockIdx.x * (256 / C) + col;
+  const bool live = j < width;
+  const uint64_t jc = live ? j : 0;
+  uint32_t stk[kDepth][8];
+#pragma unroll
+  for (int i = 0; i < M; i++) {
+    const uint4* src = (const uint4*)(cvs + ((uint64_t)(l * M + i) * width + jc) * 8);
+    const uint4 a0 = src[0], a1 = src[1];
+    uint32_t cur[8] = {a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, a1.z, a1.w};
+    const int height = __builtin_popcount(i), merges = __builtin_ctz(i + 1);
+#pragma unroll
+    for (int k = 0; k < kDepth; k++)
+      if (k < merges) parent_cv(stk[height - 1 - k], cur, false);
+#pragma unroll
+    for (int w = 0; w < 8; w++) stk[height - merges][w] = cur[w];
+  }
+  uint32_t* area = nodes + col * C * 8;
+#pragma unroll
+  for (int w = 0; w < 8; w++) area[l * 8 + w] = stk[0][w];
+  __syncthreads();
+  const uint32_t qc = l >> 2, nq = C >> 2;
+  uint32_t lo[2] = {0, 0}, hi[2] = {0, 0};
+  for (uint32_t k = C; k > 1; k >>= 1) {
+    const uint32_t merges = k >> 1;
+#pragma unroll
+    for (uint32_t r = 0; r < 2; r++) {
+      const uint32_t idx = r * nq + qc;
+      if (idx < merges) parent_quad(area + 16 * idx, q, k == 2, lo[r], hi[r]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t r = 0; r < 2; r++) {
+      const uint32_t idx = r * nq + qc;
+      if (idx < merges) {
+        area[8 * idx + q] = lo[r];
+        area[8 * idx + 4 + q] = hi[r];
+      }
+    }
+    __syncthreads();
+  }
+  // The last merge ran on the column's first quad: lane q holds root words
+  // q and 4 + q.
+  if (live && l < 4) {
+    uint32_t* dst = (uint32_t*)(leaves + j * 32);
+    dst[q] = lo[0];
+    dst[4 + q] = hi[0];
+  }
+}
+
 // Merkle layers (merkle.rs:172-221) for n = 2^h leaves stored at layers[0..n).
 // Layer L starts at node offset 2n - 2(n >> L).  A workgroup of 256 threads
 // reads 512 nodes of layer L0 and produces up to 9 layers above them in LDS,
@@ -492,11 +560,39 @@ int launch_chunk_cvs(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32
   return ZP_OK;
 }
 
+template <int M>
+int launch_cv_quad_tree(const uint32_t* cvs, uint64_t width, uint32_t C, uint8_t* leaves,
+                        hipStream_t s) {
+  const uint64_t per_block = 256 / C;
+  hipLaunchKernelGGL(cv_quad_tree_kernel<M>, dim3((uint32_t)((width + per_block - 1) / per_block)),
+                     dim3(256), 0, s, cvs, width, C, leaves);
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
+}
+
 // Merges each column's chunk chaining values (cvs, nchunks x width) up to
-// the leaf; `tmp` holds ceil(nchunks/2) x width values.
+// the leaf; `tmp` holds ceil(nchunks/2) x width values.  Power-of-two counts
+// from 4 to 2048 take the quad kernel (C lanes per column, M chunks each);
+// others the LDS tree kernel, or one launch per level when a column's tree
+// does not fit in LDS.
 int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
                     uint8_t* leaves, hipStream_t s) {
   if (nchunks <= 1) return ZP_OK;
+  if (g_force_ctree != 1 && pow2(nchunks) && nchunks >= 4 && nchunks <= 2048) {
+    // Up to 8 chunks per lane, fewer while that leaves under 2^18 lanes in
+    // all (4 waves per SIMD) and a column stays within 256 lanes.  More
+    // lanes per column add quad levels: at 16 chunks x 2^16 columns, 4 lanes
+    // of 4 chunks take 26.5 us and 8 lanes of 2 take 31.4 us.
+    uint64_t M = std::min<uint64_t>(8, nchunks / 4);
+    while (M > 1 && width * (nchunks / M) < (1u << 18) && nchunks / M < 256) M /= 2;
+    const uint32_t C = (uint32_t)(nchunks / M);
+    switch (M) {
+      case 1: return launch_cv_quad_tree<1>(cvs, width, C, leaves, s);
+      case 2: return launch_cv_quad_tree<2>(cvs, width, C, leaves, s);
+      case 4: return launch_cv_quad_tree<4>(cvs, width, C, leaves, s);
+      default: return launch_cv_quad_tree<8>(cvs, width, C, leaves, s);
+    }
+  }
   const uint32_t G = cv_tree_columns(nchunks);
   if (G > 0) {
     hipLaunchKernelGGL(cv_tree_kernel, dim3((uint32_t)((width + G - 1) / G)), dim3(256), 0, s, cvs,

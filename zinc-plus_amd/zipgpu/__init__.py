@@ -80,7 +80,9 @@ def set_option(name, value):
     1 / 2 plane groups), "commit_overlap" (1 overlap encode and hash, 0 not:
     the default), "wide_base" (Int<L>, i64 and i32 IPRS base layer: 0
     matrix cores where the code has the int8 tables, 1 per-element), "merkle_tree"
-    (0 a 4-lane quad per parent node, 1 a thread per parent).
+    (0 a 4-lane quad per parent node, 1 a thread per parent), "chunk_tree" (0
+    register folds and quad merges for 4 to 2048 chunks per column, 1 the LDS
+    tree for every count).
     """
     _check(L.load().zp_set_option(name.encode(), int(value)))
 
