@@ -135,7 +135,8 @@ def test_raa_binary_encode(zg, dev, oracle, row_len, D, rows, flip):
 
 
 @pytest.mark.parametrize("row_len,D,rows", [(2048, 64, 3), (1024, 32, 2), (256, 64, 2),
-                                             (4096, 32, 1), (512, 32, 4), (2048, 32, 2)])
+                                             (4096, 32, 1), (4096, 64, 2), (512, 32, 4),
+                                             (2048, 32, 2)])
 def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows):
     # The full-row RAA kernel (compact first scan, 32 planes of a row per
     # workgroup) and the plane-group kernel are separate device paths; both
