@@ -382,6 +382,28 @@ def test_combined_row(zg, dev, oracle, rows, row_len, D, batch, L):
     assert np.array_equal(got.view(np.uint64), want)
 
 
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_combined_row_extreme_coefficients(zg, dev, oracle, sign):
+    # Every bit set and every s_j = i128::MIN (or i128::MAX) for 256 rows:
+    # each 128-row segment sums 128 extreme words in every plane.
+    say("combined_row extreme", sign)
+    rows, row_len, D, L = 256, 64, 64, 5
+    n = rows * row_len
+    x = np.full(n, -1, dtype=np.int64)
+    coeffs = np.zeros((rows, 2), dtype=np.int64)
+    if sign < 0:
+        coeffs[:, 1] = np.iinfo(np.int64).min
+    else:
+        coeffs[:, 0] = -1
+        coeffs[:, 1] = np.iinfo(np.int64).max
+    alphas = synth.i128_words(503, D)
+    zp = zg.ZipPlus(zg.BinPolyTypes(D, comb_limbs=L))
+    pp = zp.setup(n, zg.RaaCode(row_len))
+    got = zp.combined_row(pp, [to_dev(x, dev)], to_dev(alphas, dev), to_dev(coeffs, dev))
+    want = oracle.combined_row(x.view(np.uint64), alphas, coeffs, rows, row_len, 1, D, L)
+    assert np.array_equal(got.cpu().numpy().view(np.uint64), want)
+
+
 @pytest.mark.parametrize("kind,n,row_len,D", [("iprs", 1 << 6, 8, 32), ("iprs", 1 << 8, 64, 32),
                                               ("raa", 1 << 12, 1 << 11, 32),
                                               ("iprs", 1 << 16, 1 << 14, 32),

@@ -38,6 +38,12 @@ __device__ __forceinline__ void add3(uint64_t& s0, uint64_t& s1, uint64_t& s2, u
 }
 
 // part[((b*nseg + seg)*D + c)*row_len + col][3]
+//
+// Each i128 coefficient s_j is taken as 32-bit words w0, w1, w2 (unsigned)
+// and w3 (signed), s_j = w0 + 2^32 w1 + 2^64 w2 + 2^96 w3.  Per plane the
+// thread keeps one 64-bit sum per word over its segment of at most 128 rows
+// (each below 2^39 in magnitude, so no carry is needed inside the loop) and
+// recombines the four sums into three two's-complement limbs at the end.
 __global__ void __launch_bounds__(256)
 comb_partial_kernel(const uint64_t* __restrict__ evals, const int64_t* __restrict__ coeffs,
                     uint64_t* __restrict__ part, uint64_t num_rows, uint64_t row_len, uint32_t D,
@@ -54,25 +60,39 @@ comb_partial_kernel(const uint64_t* __restrict__ evals, const int64_t* __restric
   const uint64_t j0 = (uint64_t)seg * kSegRows;
   const uint64_t j1 = min(num_rows, j0 + kSegRows);
   const uint64_t* x = evals + b * num_rows * row_len + col;
-  uint64_t s[8][3];
+  uint64_t a0[8], a1[8], a2[8];
+  int64_t a3[8];
 #pragma unroll
-  for (int k = 0; k < 8; k++) s[k][0] = s[k][1] = s[k][2] = 0;
+  for (int k = 0; k < 8; k++) a0[k] = a1[k] = a2[k] = a3[k] = 0;
   for (uint64_t j = j0; j < j1; j++) {
-    const uint32_t bits = (uint32_t)(x[j * row_len] >> (8 * g)) & 0xffu;
-    const uint64_t lo = (uint64_t)coeffs[2 * j], hi = (uint64_t)coeffs[2 * j + 1];
-    const uint64_t ext = (int64_t)hi < 0 ? ~0ull : 0ull;
+    const uint32_t bits = (uint32_t)(x[j * row_len] >> (8 * g));
+    const uint64_t lo = (uint64_t)coeffs[2 * j];
+    const int64_t hi = coeffs[2 * j + 1];
+    const uint32_t w0 = (uint32_t)lo, w1 = (uint32_t)(lo >> 32), w2 = (uint32_t)hi;
+    const int64_t w3 = hi >> 32;
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      const uint64_t m = 0ull - (uint64_t)((bits >> k) & 1u);
-      add3(s[k][0], s[k][1], s[k][2], lo & m, hi & m, ext & m);
+      const uint32_t m = 0u - ((bits >> k) & 1u);
+      a0[k] += w0 & m;
+      a1[k] += w1 & m;
+      a2[k] += w2 & m;
+      a3[k] += w3 & (int64_t)(int32_t)m;
     }
   }
   const uint32_t nc = min(8u, D - 8 * g);
-  for (uint32_t k = 0; k < nc; k++) {
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if ((uint32_t)k >= nc) break;
+    // a0 + 2^32 a1 + 2^64 a2 + 2^96 a3 as limbs s0, s1, s2 (a3 = 2^32 h + l).
+    const uint64_t s0 = a0[k] + (a1[k] << 32);
+    uint64_t s1 = (a1[k] >> 32) + a2[k] + (s0 < a0[k] ? 1u : 0u);
+    const uint64_t t = s1 + ((uint64_t)a3[k] << 32);
+    const uint64_t s2 = (uint64_t)(a3[k] >> 32) + (t < s1 ? 1u : 0u);
+    s1 = t;
     uint64_t* dst = part + ((((b * nseg + seg) * D) + 8 * g + k) * row_len + col) * 3;
-    dst[0] = s[k][0];
-    dst[1] = s[k][1];
-    dst[2] = s[k][2];
+    dst[0] = s0;
+    dst[1] = s1;
+    dst[2] = s2;
   }
 }
 
