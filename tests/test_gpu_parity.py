@@ -134,16 +134,18 @@ def test_raa_binary_encode(zg, dev, oracle, row_len, D, rows, flip):
     assert np.array_equal(got, want)
 
 
-@pytest.mark.parametrize("row_len,D,rows", [(2048, 64, 3), (1024, 32, 2), (256, 64, 2),
-                                             (4096, 32, 1), (4096, 64, 2), (512, 32, 4),
-                                             (2048, 32, 2)])
-def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows):
+@pytest.mark.parametrize("row_len,D,rows,flip", [
+    (2048, 64, 3, False), (1024, 32, 2, False), (256, 64, 2, False), (4096, 32, 1, False),
+    (4096, 64, 2, False), (512, 32, 4, False), (2048, 32, 2, False), (2048, 64, 2, True),
+    (256, 32, 3, True), (4096, 64, 1, True), (1024, 32, 2, True)])
+def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows, flip):
     # The full-row RAA kernel (compact first scan, 32 planes of a row per
-    # workgroup) and the plane-group kernel are separate device paths; both
+    # workgroup; for the sign-flipping code a second scan over the odd perm1
+    # entries) and the plane-group kernel are separate device paths; both
     # must equal the oracle.
-    say("raa kernels", row_len, D, rows)
+    say("raa kernels", row_len, D, rows, flip)
     x = synth.binary_polys(800 + row_len + D, rows * row_len, D)
-    code = zg.RaaCode(row_len)
+    code = zg.RaaSignFlippingCode(row_len) if flip else zg.RaaCode(row_len)
     types = zg.BinPolyTypes(D)
     xd = to_dev(x, dev)
     outs = []
@@ -153,7 +155,7 @@ def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows):
             outs.append(code.encode_rows(xd, rows, types).cpu().numpy())
     finally:
         zg.set_option("raa_kernel", -1)
-    want = oracle.raa_encode_binary(x.view(np.uint64), rows, row_len, D)
+    want = oracle.raa_encode_binary(x.view(np.uint64), rows, row_len, D, sign_flip=flip)
     for got in outs:
         assert np.array_equal(got, want)
 

@@ -229,7 +229,9 @@ raa_binary_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
       for (int c = 0; c < (int)PG; c += 4)
         *(uint4*)(dst + c) = make_uint4(sum[c], sum[c + 1], sum[c + 2], sum[c + 3]);
     } else {
-      for (uint32_t c = 0; c < np; c++) dst[c] = (int32_t)sum[c];
+#pragma unroll
+      for (uint32_t c = 0; c < PG; c++)
+        if (c < np) dst[c] = (int32_t)sum[c];
     }
   }
 #undef BITS
@@ -300,11 +302,19 @@ __device__ __forceinline__ uint32_t rank_word(uint32_t word, uint32_t m, uint32_
 
 // D planes per position in the output row, PG of them (planes PG*pg ..
 // PG*pg + PG - 1) per workgroup: blockIdx.x = row * (D / PG) + pg.
-template <int D>
-__global__ void __launch_bounds__(kRowThreads)
-raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
-               const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
-               uint32_t row_len, uint32_t N, uint64_t row0) {
+//
+// kFlip (the sign-flipping code): the first accumulate adds -bit where the
+// perm1 entry of the position is odd, and the accumulated value of an odd
+// position is negated.  With O the bit-vector of odd perm1 entries (one
+// ballot per 64 positions, shared by all planes), the prefix up to m is
+// rank(bits, m) - 2 rank(bits & O, m); the checkpoints hold that signed
+// prefix at word boundaries (|prefix| <= N <= 2^14 fits 16 bits).
+template <int D, bool kFlip>
+__device__ __forceinline__ void raa_row_body(const uint64_t* __restrict__ evals,
+                                             int32_t* __restrict__ out,
+                                             const uint32_t* __restrict__ perm1,
+                                             const uint32_t* __restrict__ perm2, uint32_t row_len,
+                                             uint32_t N, uint64_t row0) {
   constexpr uint32_t PG = kRowPlanes;
   constexpr uint32_t SUB = 64 / PG;          // position streams per wave
   constexpr uint32_t NS = kRowWaves * SUB;   // position streams per workgroup
@@ -318,6 +328,7 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   uint16_t* cp = (uint16_t*)(lds32 + (size_t)PG * WP);           // [PG][CP]
   uint32_t* part = lds32 + (size_t)PG * WP + ((size_t)PG * CP) / 2;
   uint16_t* p2 = (uint16_t*)(part + 2 * kRowThreads);            // [N] perm2 (N <= 2^15)
+  uint32_t* oddm = (uint32_t*)(p2 + N);                          // [W32] odd perm1 entries
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t row = row0 + blockIdx.x / NPG;
   const uint32_t poff = PG * (blockIdx.x % NPG);
@@ -342,6 +353,13 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
     for (uint32_t k = 0; k < kPre; k++) {
       const uint32_t blk = b0 + k * kRowWaves;
       if (blk >= W) break;
+      if constexpr (kFlip) {
+        const uint64_t om = __ballot((src[k] & 1u) != 0u);
+        if (lane == 0) {
+          oddm[2 * blk] = (uint32_t)om;
+          oddm[2 * blk + 1] = (uint32_t)(om >> 32);
+        }
+      }
       // Lane q collects plane poff + q's word from the q-th ballot.
       const uint64_t xq = xv[k] >> poff;
       const uint32_t lo = (uint32_t)xq, hi = (uint32_t)(xq >> 32);
@@ -369,15 +387,22 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
     const uint32_t p = tid % PG, seg = tid / PG;
     const uint32_t ws = (W32 + SEGS - 1) / SEGS;
     const uint32_t w0 = min(W32, seg * ws), w1 = min(W32, w0 + ws);
+    // Signed word counts of the flipped code wrap mod 2^32 like the
+    // plain ones; every prefix fits 16 bits.
+    auto count = [&](uint32_t w) -> uint32_t {
+      const uint32_t b = bits[p * WP + w];
+      if constexpr (kFlip) return __popc(b) - 2u * __popc(b & oddm[w]);
+      return __popc(b);
+    };
     uint32_t cnt = 0;
-    for (uint32_t w = w0; w < w1; w++) cnt += __popc(bits[p * WP + w]);
+    for (uint32_t w = w0; w < w1; w++) cnt += count(w);
     part[tid] = cnt;
     __syncthreads();
     uint32_t before = 0;
     for (uint32_t sg = 0; sg < seg; sg++) before += part[sg * PG + p];
     for (uint32_t w = w0; w < w1; w++) {
       cp[p * CP + w] = (uint16_t)before;
-      before += __popc(bits[p * WP + w]);
+      before += count(w);
     }
   }
   __syncthreads();
@@ -394,6 +419,12 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   const uint16_t* cpp = cp + p * CP;
   auto rank = [&](uint32_t m) -> uint32_t {
     const uint32_t w = m >> 5;
+    if constexpr (kFlip) {
+      const uint32_t all = rank_word(bp[w], m, (uint32_t)(int32_t)(int16_t)cpp[w]);
+      const uint32_t odd = rank_word(bp[w] & oddm[w], m, 0u);
+      const uint32_t v = all - 2u * odd;
+      return (m & 1u) ? 0u - v : v;
+    }
     return rank_word(bp[w], m, cpp[w]);
   };
   int32_t* orow = out + row * N * D + poff + p;
@@ -429,27 +460,47 @@ raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
   }
 }
 
+template <int D>
+__global__ void __launch_bounds__(kRowThreads)
+raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
+               const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
+               uint32_t row_len, uint32_t N, uint64_t row0) {
+  raa_row_body<D, false>(evals, out, perm1, perm2, row_len, N, row0);
+}
+
+// The flipped lookups need more registers; 4 waves per SIMD keep two
+// workgroups of 8 waves on a CU.
+template <int D>
+__global__ void __launch_bounds__(kRowThreads, 4)
+raa_row_flip_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
+                    const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
+                    uint32_t row_len, uint32_t N, uint64_t row0) {
+  raa_row_body<D, true>(evals, out, perm1, perm2, row_len, N, row0);
+}
+
 // bits, checkpoints, the exchange area (phase-1b partial counts, then two
 // buffers of per-stream totals: 2 * 8 * 64 words), and perm2.
 constexpr uint32_t kRowPartWords = 2 * kRowThreads;
 
-size_t raa_row_lds(uint32_t N) {
+size_t raa_row_lds(uint32_t N, bool flip) {
   return (size_t)kRowPlanes * raa_bits_stride(N) * 4 +
-         (size_t)kRowPlanes * raa_cp_stride(N) * 2 + (size_t)kRowPartWords * 4 + (size_t)N * 2;
+         (size_t)kRowPlanes * raa_cp_stride(N) * 2 + (size_t)kRowPartWords * 4 + (size_t)N * 2 +
+         (flip ? (size_t)N / 8 : 0);
 }
 
-template <int D>
+template <int D, bool kFlip>
 int launch_raa_row(const zp_code* cd, const uint64_t* evals, uint64_t rows, int32_t* out,
                    hipStream_t s) {
   const uint32_t N = (uint32_t)cd->N;
-  const size_t lds = raa_row_lds(N);
-  ZP_TRY(grant_lds<raa_row_kernel<D>>(lds));
+  const size_t lds = raa_row_lds(N, kFlip);
+  constexpr auto kernel = kFlip ? raa_row_flip_kernel<D> : raa_row_kernel<D>;
+  ZP_TRY(grant_lds<kernel>(lds));
   constexpr uint32_t NPG = D / kRowPlanes;
   const uint64_t max_rows = 0x7fffffffull / NPG;
   for (uint64_t r0 = 0; r0 < rows; r0 += max_rows) {
     const uint32_t nr = (uint32_t)std::min<uint64_t>(max_rows, rows - r0);
-    hipLaunchKernelGGL((raa_row_kernel<D>), dim3(nr * NPG), dim3(kRowThreads), lds, s, evals,
-                       out, cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, r0);
+    hipLaunchKernelGGL(kernel, dim3(nr * NPG), dim3(kRowThreads), lds, s, evals, out,
+                       cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, r0);
     ZP_HIP(hipGetLastError());
   }
   return ZP_OK;
@@ -476,10 +527,12 @@ int dispatch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32
 }  // namespace
 
 bool raa_full_row_eligible(const zp_code* cd, uint32_t D) {
-  // Plain code, 32 or 64 planes, every stream a whole number of positions,
-  // and the bit-vectors within LDS.
-  return cd->kind == ZP_CODE_RAA && (D == 64 || D == 32) && cd->N >= 1024 &&
-         cd->N <= (1u << 15) && raa_row_lds((uint32_t)cd->N) <= 150 * 1024;
+  // Either code, 32 or 64 planes, every stream a whole number of positions,
+  // and the bit-vectors within LDS (N <= 2^14 here, so a flipped prefix fits
+  // the 16-bit checkpoints).
+  const bool flip = cd->kind == ZP_CODE_RAA_SIGNFLIP;
+  return (cd->kind == ZP_CODE_RAA || flip) && (D == 64 || D == 32) && cd->N >= 1024 &&
+         cd->N <= (1u << 14) && raa_row_lds((uint32_t)cd->N, flip) <= 150 * 1024;
 }
 
 // variant: 0 full-row kernel where eligible, 1 plane groups of 8 (at
@@ -489,9 +542,13 @@ int launch_raa_binary(const zp_code* cd, const uint64_t* evals, uint64_t rows, u
   if (rows == 0) return ZP_OK;
   if (cd->N > (1u << 15))
     return fail(ZP_ERR_INVALID_PARAM, "RAA codeword length above 32768 is not supported");
-  if (variant == 0 && raa_full_row_eligible(cd, D))
-    return D == 64 ? launch_raa_row<64>(cd, evals, rows, out, s)
-                   : launch_raa_row<32>(cd, evals, rows, out, s);
+  if (variant == 0 && raa_full_row_eligible(cd, D)) {
+    if (cd->kind == ZP_CODE_RAA_SIGNFLIP)
+      return D == 64 ? launch_raa_row<64, true>(cd, evals, rows, out, s)
+                     : launch_raa_row<32, true>(cd, evals, rows, out, s);
+    return D == 64 ? launch_raa_row<64, false>(cd, evals, rows, out, s)
+                   : launch_raa_row<32, false>(cd, evals, rows, out, s);
+  }
   const bool wide = variant != 2;
   return cd->kind == ZP_CODE_RAA_SIGNFLIP ? dispatch_raa<true>(cd, evals, rows, D, out, s, wide)
                                           : dispatch_raa<false>(cd, evals, rows, D, out, s, wide);
