@@ -13,7 +13,8 @@ Also reported on the same line:
                 committed rocprofv3 counter summary when present
   cpu_baseline  the CPU oracle committing a bounded sample of the workload
   workloads     plain Zip 2^16 i32 commit; RAA 2^22 BinaryPoly<64> encode,
-                commit and combined row; 64 mixed commits as one batch plan
+                commit and combined row; the sign-flipping RAA code's encode
+                at the same shape; 64 mixed commits as one batch plan
                 (HIP graph) against the plain per-poly path; the commit/open
                 service under Poisson arrivals (p50/p99 latency) against one
                 request at a time; both IPRS encoders at the headline shape.
@@ -154,6 +155,22 @@ def leg_raa(torch, zipgpu, synth, dev):
             "encode_ms": enc, "encode_gbs_algorithmic": enc_gbs,
             "encode_hbm_frac": enc_gbs / HBM_PEAK_GBS, "commit_ms": ms,
             "evals_per_s": n / ms * 1e3, "combined_row_ms": comb}
+
+
+def leg_raa_signflip(torch, zipgpu, synth, dev):
+    """RaaSignFlippingCode encode_rows: 2^22 BinaryPoly<64>, row 2^11 x 2^11 rows."""
+    n, row_len, d = 1 << 22, 1 << 11, 64
+    x = torch.from_numpy(synth.binary_polys(3, n, d)).to(dev)
+    code = zipgpu.RaaSignFlippingCode(row_len)
+    types = zipgpu.BinPolyTypes(d)
+    pp = zipgpu.ZipPlus(types).setup(n, code)
+    out = torch.empty((pp.num_rows, code.codeword_len(), d), dtype=torch.int32, device=dev)
+    stream = torch.cuda.Stream()
+    enc = event_ms(torch, lambda s: code.encode_rows(x, pp.num_rows, types, out=out, stream=s),
+                   stream, 20, warmup=5)
+    gbs = (8 + REP * d * 4) * n / (enc * 1e-3) / 1e9
+    return {"workload": "RaaSignFlippingCode encode_rows 2^22 BinaryPoly<64>, row 2^11 x 2^11",
+            "encode_ms": enc, "encode_gbs_algorithmic": gbs, "encode_hbm_frac": gbs / HBM_PEAK_GBS}
 
 
 def leg_encode_wide(torch, zipgpu, synth, dev):
@@ -537,6 +554,7 @@ def main():
     if not args.no_legs:
         legs = {}
         for name, fn in (("plain_zip_2p16_i32", leg_plain_zip), ("raa_2p22_d64", leg_raa),
+                         ("raa_signflip_2p22_d64", leg_raa_signflip),
                          ("batched_64_mixed", leg_batched), ("served_commit_open", leg_served),
                          ("encode_wide_int5", leg_encode_wide),
                          ("prove", lambda *a: leg_prove(*a, cpu=not args.no_cpu)),
