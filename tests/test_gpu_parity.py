@@ -160,6 +160,32 @@ def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows, flip):
         assert np.array_equal(got, want)
 
 
+@pytest.mark.parametrize("flip", [False, True])
+def test_raa_full_size_kernels_and_linearity(zg, dev, flip):
+    # 2^22 BinaryPoly<64>, row 2^11 x 2^11 rows: the full-row kernel equals
+    # the plane-group kernel, and for bitwise-disjoint a, b the codewords
+    # add (RAA is linear over the integers and every entry fits i32).
+    say("raa full size", flip)
+    n, row_len, D = 1 << 22, 1 << 11, 64
+    x = synth.binary_polys(41, n, D)
+    m = synth.binary_polys(42, n, D)
+    a, b = x & m, x & ~m
+    code = zg.RaaSignFlippingCode(row_len) if flip else zg.RaaCode(row_len)
+    types = zg.BinPolyTypes(D)
+    rows = n // row_len
+    try:
+        zg.set_option("raa_kernel", 1)
+        groups = code.encode_rows(to_dev(x, dev), rows, types)
+        zg.set_option("raa_kernel", 0)
+        full = code.encode_rows(to_dev(x, dev), rows, types)
+        ea = code.encode_rows(to_dev(a, dev), rows, types)
+        eb = code.encode_rows(to_dev(b, dev), rows, types)
+    finally:
+        zg.set_option("raa_kernel", -1)
+    assert torch.equal(full, groups)
+    assert torch.equal(ea + eb, full)
+
+
 def test_raa_width_assert(zg, dev):
     # raa.rs:52-69: 1 + 2*14 + 2*2 = 33 bits do not fit i32 codewords.
     code = zg.RaaCode(1 << 14)
