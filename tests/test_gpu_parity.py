@@ -137,7 +137,9 @@ def test_raa_binary_encode(zg, dev, oracle, row_len, D, rows, flip):
 @pytest.mark.parametrize("row_len,D,rows,flip", [
     (2048, 64, 3, False), (1024, 32, 2, False), (256, 64, 2, False), (4096, 32, 1, False),
     (4096, 64, 2, False), (512, 32, 4, False), (2048, 32, 2, False), (2048, 64, 2, True),
-    (256, 32, 3, True), (4096, 64, 1, True), (1024, 32, 2, True)])
+    (256, 32, 3, True), (4096, 64, 1, True), (1024, 32, 2, True), (2048, 3, 2, False),
+    (1024, 16, 3, False), (2048, 48, 2, True), (256, 1, 2, True), (4096, 40, 1, False),
+    (2048, 17, 2, True)])
 def test_raa_device_kernels_agree(zg, dev, oracle, row_len, D, rows, flip):
     # The full-row RAA kernel (compact first scan, 32 planes of a row per
     # workgroup; for the sign-flipping code a second scan over the odd perm1

@@ -301,7 +301,9 @@ __device__ __forceinline__ uint32_t rank_word(uint32_t word, uint32_t m, uint32_
 }
 
 // D planes per position in the output row, PG of them (planes PG*pg ..
-// PG*pg + PG - 1) per workgroup: blockIdx.x = row * (D / PG) + pg.
+// PG*pg + PG - 1) per workgroup: blockIdx.x = row * ceil(D / PG) + pg.
+// kD = 32 or 64 fixes D at compile time; kD = 0 takes D (16..64) at run
+// time, and lanes of planes at or above D compute but do not store.
 //
 // kFlip (the sign-flipping code): the first accumulate adds -bit where the
 // perm1 entry of the position is odd, and the accumulated value of an odd
@@ -309,16 +311,17 @@ __device__ __forceinline__ uint32_t rank_word(uint32_t word, uint32_t m, uint32_
 // ballot per 64 positions, shared by all planes), the prefix up to m is
 // rank(bits, m) - 2 rank(bits & O, m); the checkpoints hold that signed
 // prefix at word boundaries (|prefix| <= N <= 2^14 fits 16 bits).
-template <int D, bool kFlip>
+template <int kD, bool kFlip>
 __device__ __forceinline__ void raa_row_body(const uint64_t* __restrict__ evals,
                                              int32_t* __restrict__ out,
                                              const uint32_t* __restrict__ perm1,
                                              const uint32_t* __restrict__ perm2, uint32_t row_len,
-                                             uint32_t N, uint64_t row0) {
+                                             uint32_t N, uint64_t row0, uint32_t Drt) {
   constexpr uint32_t PG = kRowPlanes;
+  const uint32_t D = kD ? (uint32_t)kD : Drt;
   constexpr uint32_t SUB = 64 / PG;          // position streams per wave
   constexpr uint32_t NS = kRowWaves * SUB;   // position streams per workgroup
-  constexpr uint32_t NPG = D / PG;
+  const uint32_t NPG = kD ? (uint32_t)kD / PG : (D + PG - 1) / PG;
   extern __shared__ uint32_t lds32[];
   const uint32_t W = N / 64;                 // 64-bit words (ballots) per plane
   const uint32_t W32 = N / 32;               // 32-bit words per plane
@@ -428,6 +431,7 @@ __device__ __forceinline__ void raa_row_body(const uint64_t* __restrict__ evals,
     return rank_word(bp[w], m, cpp[w]);
   };
   int32_t* orow = out + row * N * D + poff + p;
+  const bool stores = kD != 0 || poff + p < D;
   uint32_t run = 0;  // every position of the earlier steps
   for (uint32_t st = 0; st < steps; st++) {
     const uint32_t base = (st * NS + s) * K;
@@ -454,28 +458,30 @@ __device__ __forceinline__ void raa_row_body(const uint64_t* __restrict__ evals,
       before += q < s ? t : 0u;
     }
     const uint32_t c = run + before;
+    if (stores) {
 #pragma unroll
-    for (uint32_t k = 0; k < K; k++) orow[(uint64_t)(base + k) * D] = (int32_t)(c + v[k]);
+      for (uint32_t k = 0; k < K; k++) orow[(uint64_t)(base + k) * D] = (int32_t)(c + v[k]);
+    }
     run += all;
   }
 }
 
-template <int D>
+template <int kD>
 __global__ void __launch_bounds__(kRowThreads)
 raa_row_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
                const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
-               uint32_t row_len, uint32_t N, uint64_t row0) {
-  raa_row_body<D, false>(evals, out, perm1, perm2, row_len, N, row0);
+               uint32_t row_len, uint32_t N, uint64_t row0, uint32_t D) {
+  raa_row_body<kD, false>(evals, out, perm1, perm2, row_len, N, row0, D);
 }
 
 // The flipped lookups need more registers; 4 waves per SIMD keep two
 // workgroups of 8 waves on a CU.
-template <int D>
+template <int kD>
 __global__ void __launch_bounds__(kRowThreads, 4)
 raa_row_flip_kernel(const uint64_t* __restrict__ evals, int32_t* __restrict__ out,
                     const uint32_t* __restrict__ perm1, const uint32_t* __restrict__ perm2,
-                    uint32_t row_len, uint32_t N, uint64_t row0) {
-  raa_row_body<D, true>(evals, out, perm1, perm2, row_len, N, row0);
+                    uint32_t row_len, uint32_t N, uint64_t row0, uint32_t D) {
+  raa_row_body<kD, true>(evals, out, perm1, perm2, row_len, N, row0, D);
 }
 
 // bits, checkpoints, the exchange area (phase-1b partial counts, then two
@@ -488,19 +494,19 @@ size_t raa_row_lds(uint32_t N, bool flip) {
          (flip ? (size_t)N / 8 : 0);
 }
 
-template <int D, bool kFlip>
-int launch_raa_row(const zp_code* cd, const uint64_t* evals, uint64_t rows, int32_t* out,
-                   hipStream_t s) {
+template <int kD, bool kFlip>
+int launch_raa_row(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
+                   int32_t* out, hipStream_t s) {
   const uint32_t N = (uint32_t)cd->N;
   const size_t lds = raa_row_lds(N, kFlip);
-  constexpr auto kernel = kFlip ? raa_row_flip_kernel<D> : raa_row_kernel<D>;
+  constexpr auto kernel = kFlip ? raa_row_flip_kernel<kD> : raa_row_kernel<kD>;
   ZP_TRY(grant_lds<kernel>(lds));
-  constexpr uint32_t NPG = D / kRowPlanes;
+  const uint32_t NPG = (D + kRowPlanes - 1) / kRowPlanes;
   const uint64_t max_rows = 0x7fffffffull / NPG;
   for (uint64_t r0 = 0; r0 < rows; r0 += max_rows) {
     const uint32_t nr = (uint32_t)std::min<uint64_t>(max_rows, rows - r0);
     hipLaunchKernelGGL(kernel, dim3(nr * NPG), dim3(kRowThreads), lds, s, evals, out,
-                       cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, r0);
+                       cd->d_perm1, cd->d_perm2, (uint32_t)cd->row_len, N, r0, D);
     ZP_HIP(hipGetLastError());
   }
   return ZP_OK;
@@ -527,11 +533,13 @@ int dispatch_raa(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32
 }  // namespace
 
 bool raa_full_row_eligible(const zp_code* cd, uint32_t D) {
-  // Either code, 32 or 64 planes, every stream a whole number of positions,
+  // Either code, 16 to 64 planes, every stream a whole number of positions,
   // and the bit-vectors within LDS (N <= 2^14 here, so a flipped prefix fits
-  // the 16-bit checkpoints).
+  // the 16-bit checkpoints).  Below 16 planes most lanes of a 32-plane
+  // workgroup would idle; the plane-group kernel is as fast or faster there
+  // (2^22 evals, D = 1: 0.17 against 0.31 ms; D = 16: 0.74 against 0.40 ms).
   const bool flip = cd->kind == ZP_CODE_RAA_SIGNFLIP;
-  return (cd->kind == ZP_CODE_RAA || flip) && (D == 64 || D == 32) && cd->N >= 1024 &&
+  return (cd->kind == ZP_CODE_RAA || flip) && D >= 16 && D <= 64 && cd->N >= 1024 &&
          cd->N <= (1u << 14) && raa_row_lds((uint32_t)cd->N, flip) <= 150 * 1024;
 }
 
@@ -543,11 +551,14 @@ int launch_raa_binary(const zp_code* cd, const uint64_t* evals, uint64_t rows, u
   if (cd->N > (1u << 15))
     return fail(ZP_ERR_INVALID_PARAM, "RAA codeword length above 32768 is not supported");
   if (variant == 0 && raa_full_row_eligible(cd, D)) {
-    if (cd->kind == ZP_CODE_RAA_SIGNFLIP)
-      return D == 64 ? launch_raa_row<64, true>(cd, evals, rows, out, s)
-                     : launch_raa_row<32, true>(cd, evals, rows, out, s);
-    return D == 64 ? launch_raa_row<64, false>(cd, evals, rows, out, s)
-                   : launch_raa_row<32, false>(cd, evals, rows, out, s);
+    if (cd->kind == ZP_CODE_RAA_SIGNFLIP) {
+      if (D == 64) return launch_raa_row<64, true>(cd, evals, rows, D, out, s);
+      if (D == 32) return launch_raa_row<32, true>(cd, evals, rows, D, out, s);
+      return launch_raa_row<0, true>(cd, evals, rows, D, out, s);
+    }
+    if (D == 64) return launch_raa_row<64, false>(cd, evals, rows, D, out, s);
+    if (D == 32) return launch_raa_row<32, false>(cd, evals, rows, D, out, s);
+    return launch_raa_row<0, false>(cd, evals, rows, D, out, s);
   }
   const bool wide = variant != 2;
   return cd->kind == ZP_CODE_RAA_SIGNFLIP ? dispatch_raa<true>(cd, evals, rows, D, out, s, wide)
