@@ -50,10 +50,12 @@ def test_iprs_binary_ignores_bits_above_degree(zg, dev, oracle):
 
 
 @pytest.mark.parametrize("row_len,depth,D", [
-    (256, 1, 32), (2048, 1, 64), (1 << 13, 2, 32), (1 << 14, 2, 64), (1 << 12, 2, 16)])
+    (256, 1, 32), (2048, 1, 64), (1 << 13, 2, 32), (1 << 14, 2, 64), (1 << 12, 2, 16),
+    (1 << 14, 2, 20), (256, 1, 8), (2048, 1, 40), (1 << 13, 2, 1), (1 << 12, 2, 63)])
 def test_iprs_device_encoders_agree(zg, dev, oracle, row_len, depth, D):
     # The matrix-core encoder and the VALU encoder are separate device paths;
-    # both must equal the oracle.
+    # both must equal the oracle.  D not a multiple of 16 pads the plane
+    # words and skips the stores of the extra planes.
     say("encoders", row_len, depth, D)
     rows = 2
     x = synth.binary_polys(600 + row_len + D, rows * row_len, D)

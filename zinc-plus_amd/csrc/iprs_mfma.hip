@@ -187,13 +187,16 @@ __device__ __forceinline__ void fix_odd(double (&o)[8], uint32_t mask, const dou
       if ((mask >> (jo * 4 + t)) & 1u) o[t + 4] = __builtin_fma(-65537.0, x[2 * jo + 1], o[t + 4]);
 }
 
-template <int NC, int S>
+// D is the plane-word layout's plane count (a multiple of 16) and Dout the
+// codeword's; with kPart (Dout not a multiple of 16) the lanes of planes at
+// or above Dout compute but do not store.
+template <int NC, int S, bool kPart>
 __global__ void __launch_bounds__(256, 2)
 iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ afrag,
                  const double* __restrict__ tw0, const double* __restrict__ tw1,
                  const uint16_t* __restrict__ q0, const uint16_t* __restrict__ q1,
-                 int64_t* __restrict__ out, uint32_t bd, uint32_t D, uint32_t nrb, uint32_t npg,
-                 uint64_t row0) {
+                 int64_t* __restrict__ out, uint32_t bd, uint32_t D, uint32_t Dout, uint32_t nrb,
+                 uint32_t npg, uint64_t row0) {
   // Base sums [c][r_local][p], from the matrix cores to the stage threads.
   constexpr uint32_t kBsWords = NC * 16 * 16;
   __shared__ int32_t Bs[kBsWords];
@@ -283,9 +286,10 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   const uint64_t N = (uint64_t)bd * NC;
   // One row's codewords span N*D*8 <= 32 MiB, so 32-bit element offsets from
   // the row base address every output.
-  char* orow = (char*)(out + (row0 + row) * N * D);
-  const uint32_t obase = (r * D + 16 * pg + p) * 8;  // byte offset of chunk 0
-  const uint32_t cstride = bd * D * 8;               // byte stride between chunks
+  char* orow = (char*)(out + (row0 + row) * N * Dout);
+  const uint32_t obase = (r * Dout + 16 * pg + p) * 8;  // byte offset of chunk 0
+  const uint32_t cstride = bd * Dout * 8;               // byte stride between chunks
+  const bool stores = !kPart || 16 * pg + p < Dout;
 
   // Stage 1's first twiddles are loaded before stage 0, which hides them.
   double t1n[2];
@@ -325,9 +329,11 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   }
 
   if constexpr (NC == 8) {
+    if (stores) {
 #pragma unroll
-    for (int c = 0; c < 8; c++)
-      *(int64_t*)(orow + (obase + (uint32_t)c * cstride)) = dbl_to_i64(y0[c]);
+      for (int c = 0; c < 8; c++)
+        *(int64_t*)(orow + (obase + (uint32_t)c * cstride)) = dbl_to_i64(y0[c]);
+    }
   } else {
     // Stage 1: output chunk 8t + cl from the inputs 8j + cl.  The next
     // step's twiddles are loaded before this step's stores: the memory
@@ -386,7 +392,7 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
       for (int t = 0; t < 8; t++) {
         const uint32_t hlo = (uint32_t)__double_as_longlong(sh[t]);
         const uint64_t lb = (uint64_t)__double_as_longlong(sl[t]);
-        *(uint2*)(orow + ooff) = make_uint2((uint32_t)lb, (uint32_t)(lb >> 32) + hlo);
+        if (stores) *(uint2*)(orow + ooff) = make_uint2((uint32_t)lb, (uint32_t)(lb >> 32) + hlo);
         ooff += 8 * cstride;
       }
     }
@@ -396,14 +402,21 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
 template <int NC, int S>
 int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32_t D,
                 int64_t* out, hipStream_t s) {
-  const uint32_t nrb = cd->bd / 16, npg = D / 16;
+  const uint32_t Dp = (D + 15) / 16 * 16;  // plane-word layout
+  const uint32_t nrb = cd->bd / 16, npg = Dp / 16;
   const uint64_t per_row = (uint64_t)nrb * npg;
   const uint64_t max_rows = std::max<uint64_t>(1, 0x7fffffffull / per_row);
   for (uint64_t r0 = 0; r0 < rows; r0 += max_rows) {
     const uint64_t nr = std::min(max_rows, rows - r0);
-    hipLaunchKernelGGL((iprs_mfma_kernel<NC, S>), dim3((uint32_t)(nr * per_row)), dim3(256), 0, s,
-                       planes + r0 * NC * 4 * S * D, (const v4i*)cd->d_afrag, cd->d_tw0f,
-                       cd->d_tw1f, cd->d_q0f, cd->d_q1f, out, cd->bd, D, nrb, npg, r0);
+    const uint32_t* pl = planes + r0 * NC * 4 * S * Dp;
+    if (D % 16 == 0)
+      hipLaunchKernelGGL((iprs_mfma_kernel<NC, S, false>), dim3((uint32_t)(nr * per_row)),
+                         dim3(256), 0, s, pl, (const v4i*)cd->d_afrag, cd->d_tw0f, cd->d_tw1f,
+                         cd->d_q0f, cd->d_q1f, out, cd->bd, Dp, D, nrb, npg, r0);
+    else
+      hipLaunchKernelGGL((iprs_mfma_kernel<NC, S, true>), dim3((uint32_t)(nr * per_row)),
+                         dim3(256), 0, s, pl, (const v4i*)cd->d_afrag, cd->d_tw0f, cd->d_tw1f,
+                         cd->d_q0f, cd->d_q1f, out, cd->bd, Dp, D, nrb, npg, r0);
     ZP_HIP(hipGetLastError());
   }
   return ZP_OK;
@@ -413,12 +426,13 @@ template <int NC>
 int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D, int64_t* out,
               hipStream_t s) {
   const uint32_t S = cd->mfma_ksteps, W = 4 * S;
-  const uint64_t bytes = rows * NC * W * D * sizeof(uint32_t);
+  const uint32_t Dp = (D + 15) / 16 * 16;  // planes above D are computed, not stored
+  const uint64_t bytes = rows * NC * W * Dp * sizeof(uint32_t);
   uint32_t* planes = nullptr;
   ZP_TRY(dev_alloc((void**)&planes, bytes, s));
-  const uint64_t threads = rows * NC * W * (D / kPlanesPerLane);
+  const uint64_t threads = rows * NC * W * (Dp / kPlanesPerLane);
   hipLaunchKernelGGL((bitplane_kernel<NC>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,
-                     s, evals, planes, (uint32_t)cd->row_len, cd->bl, W, D, cd->depth, rows);
+                     s, evals, planes, (uint32_t)cd->row_len, cd->bl, W, Dp, cd->depth, rows);
   int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "bitplane launch");
   if (rc == ZP_OK) {
     switch (S) {
@@ -436,7 +450,7 @@ int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t 
 }  // namespace
 
 bool iprs_mfma_eligible(const zp_code* cd, uint32_t D) {
-  return cd->d_afrag != nullptr && (D % 16) == 0 && (cd->depth == 1 || cd->depth == 2) &&
+  return cd->d_afrag != nullptr && D >= 1 && D <= 64 && (cd->depth == 1 || cd->depth == 2) &&
          cd->row_len < (1ull << 31);
 }
 
