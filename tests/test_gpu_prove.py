@@ -91,6 +91,8 @@ def run_case(zg, dev, oracle, kind, row_len, num_rows, D, batch, L, eval_type, p
     ("iprs", 256, 8, 7, 3, 8, "binpoly", "random"),
     ("iprs", 1024, 4, 1, 1, 3, "i32", "random"),
     ("iprs", 1 << 14, 4, 1, 1, 3, "i32", "ones"),
+    # 147 openings of 128 KiB: the proof copies back in two pieces.
+    ("raa", 1 << 11, 512, 64, 1, 5, "binpoly", "ones"),
 ])
 def test_prove_matches_oracle(zg, dev, oracle, kind, row_len, num_rows, D, batch, L, eval_type,
                               point_kind):

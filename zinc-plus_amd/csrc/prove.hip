@@ -477,16 +477,42 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
                        (uint32_t*)d_open, rows, N, (uint32_t)(elem / 4), levels, num_openings);
     if (hipGetLastError() != hipSuccess) return done(fail(ZP_ERR_DEVICE, "launch failed"));
   }
+  // Proof bytes: the combined row (write_const_many), then the openings.
+  // The openings come back in pieces, each followed by an event, so the
+  // host appends piece k to the stream while later pieces are in flight.
   uint8_t* o_comb = pin + up2;
   uint8_t* o_open = o_comb + row_len * L * 8;
-  if (hipMemcpyAsync(o_comb, d_comb, row_len * L * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
-      (num_openings && hipMemcpyAsync(o_open, d_open, (uint64_t)num_openings * open_bytes,
-                                      hipMemcpyDeviceToHost, s) != hipSuccess) ||
-      hipStreamSynchronize(s) != hipSuccess)
-    return done(fail(ZP_ERR_DEVICE, "proof copy failed"));
-
-  // Proof bytes: the combined row (write_const_many), then the openings.
-  ts_reserve(t, down2);
-  ts_put(t, o_comb, down2);
-  return done(ZP_OK);
+  const uint64_t comb_bytes = row_len * L * 8;
+  const uint64_t open_total = (uint64_t)num_openings * open_bytes;
+  constexpr uint64_t kPiece = 16ull << 20;
+  const uint64_t npieces = (open_total + kPiece - 1) / kPiece;
+  std::vector<hipEvent_t> pev(npieces, nullptr);
+  rc = hipMemcpyAsync(o_comb, d_comb, comb_bytes, hipMemcpyDeviceToHost, s) == hipSuccess
+           ? ZP_OK
+           : fail(ZP_ERR_DEVICE, "proof copy failed");
+  for (uint64_t k = 0; rc == ZP_OK && k < npieces; k++) {
+    const uint64_t off = k * kPiece, len = std::min(kPiece, open_total - off);
+    if (hipEventCreateWithFlags(&pev[k], hipEventDisableTiming) != hipSuccess ||
+        hipMemcpyAsync(o_open + off, (const uint8_t*)d_open + off, len, hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipEventRecord(pev[k], s) != hipSuccess)
+      rc = fail(ZP_ERR_DEVICE, "proof copy failed");
+  }
+  if (rc == ZP_OK && npieces == 0 && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "proof copy failed");
+  if (rc == ZP_OK) ts_reserve(t, down2);
+  for (uint64_t k = 0; rc == ZP_OK && k < npieces; k++) {
+    if (hipEventSynchronize(pev[k]) != hipSuccess) {
+      rc = fail(ZP_ERR_DEVICE, "proof copy failed");
+      break;
+    }
+    if (k == 0) ts_put(t, o_comb, comb_bytes);
+    const uint64_t off = k * kPiece;
+    ts_put(t, o_open + off, std::min(kPiece, open_total - off));
+  }
+  if (rc == ZP_OK && npieces == 0) ts_put(t, o_comb, comb_bytes);
+  if (rc != ZP_OK) (void)hipStreamSynchronize(s);  // no copy may still target `pin`
+  for (hipEvent_t e : pev)
+    if (e) (void)hipEventDestroy(e);
+  return done(rc);
 }
