@@ -22,6 +22,15 @@
  * it panics on an assert it returns ZP_ERR_SHAPE.  zp_last_error() gives the
  * message of the calling thread's last failure.
  *
+ * Arithmetic follows the reference's unchecked build (the `unchecked`
+ * feature of its benches, CHECK = false): codeword and combined-row values
+ * are exact modulo their types, i.e. they wrap where a checked build would
+ * panic.  The path's binary-poly, i32 and i64 codes cannot overflow at
+ * depth <= 2 (every new_with_optimal_depth code); an explicit depth-3 IPRS
+ * code over binary polys, and IPRS over DensePolynomial<i64, D> with
+ * coefficients above about 2^40, can.  ZP_ERR_OVERFLOW is reserved for a
+ * checked mode and not returned.
+ *
  * Thread safety: codes are immutable after creation and may be shared by any
  * number of threads; hints are owned by one caller at a time.
  */
@@ -104,6 +113,17 @@ uint64_t zp_code_codeword_len(const zp_code* code);
 uint32_t zp_code_depth(const zp_code* code);
 int zp_code_kind_of(const zp_code* code);
 
+/* validate_input's bit budget of the linear combinations, the assert of
+ * zip-plus/src/pcs/utils.rs:24-54 that ZipPlus::commit (phase_commit.rs:
+ * 70-77) and prove run before any work: ZP_ERR_SHAPE exactly where that
+ * assert panics, for a code of repetition factor `rep` and row length
+ * `row_len` (Lc::REPETITION_FACTOR, row_len), CombR = Int<comb_limbs>,
+ * Chal = i128 and `batch` polys.  zp_commit carries no CombR, so a caller
+ * that mirrors ZipPlus::commit calls this first; zp_prove applies it
+ * itself.  Host only: needs no device. */
+int zp_validate_input(uint32_t rep, uint64_t row_len, int eval_type, uint32_t degree_plus_one,
+                      uint32_t comb_limbs, uint32_t batch);
+
 /* Bytes of one codeword element for an eval type (16 for i128, 8*D or 4*D
  * for binary polys under IPRS / RAA). */
 uint64_t zp_cw_elem_bytes(const zp_code* code, int eval_type, uint32_t degree_plus_one);
@@ -155,6 +175,18 @@ int zp_merkle_prove(const zp_hint* hint, uint64_t leaf, uint8_t* h_siblings, uin
 /* Column `col` over all rows of all polys (the bytes a column opening sends,
  * zip-plus/src/pcs/phase_prove.rs:247-304): batch*num_rows elements. */
 int zp_hint_open_column(const zp_hint* hint, uint64_t col, void* h_out, zp_stream stream);
+/* Records that work on `stream` uses the hint's device memory (reads through
+ * zp_hint_codewords, or any call given a stream other than the commit's).
+ * No reference counterpart: the reference's hint is host memory.  Without
+ * it, zp_hint_free returns the memory to the commit stream's cache at once,
+ * and the next commit on that stream may overwrite it while work on another
+ * stream still reads it.  With it, the free records an event on each such
+ * stream, and the next user of the memory waits for those events first
+ * (the rule of torch's Tensor.record_stream). */
+int zp_hint_record_stream(zp_hint* hint, zp_stream stream);
+/* Releases the hint.  Its memory goes back to the library's cache of the
+ * commit stream; work on other streams that reads it must be recorded with
+ * zp_hint_record_stream, or have finished. */
 void zp_hint_free(zp_hint* hint);
 
 /* ---- Batched commitments ------------------------------------------------

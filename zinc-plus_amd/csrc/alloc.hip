@@ -2,14 +2,19 @@
 //
 // Blocks are cached per (device, stream).  A block freed on stream s is only
 // handed out again to work on the same stream s, so stream order makes the
-// reuse safe without events or synchronisation.  Sizes are rounded up (2 MiB
-// granules above 1 MiB) so that repeated commits of one shape hit the cache.
-// When hipMalloc fails, the cache of that device is released (after a device
-// synchronisation) and the allocation is retried once.
+// reuse safe without synchronisation.  A block that was also used on other
+// streams (dev_free_after) carries one event per such stream, recorded at
+// the free; the next allocation that takes it makes its stream wait for
+// those events first, so work on the other streams that was enqueued before
+// the free finishes before the block is written again.  Sizes are rounded up
+// (2 MiB granules above 1 MiB) so that repeated commits of one shape hit the
+// cache.  When hipMalloc fails, the cache of that device is released (after
+// a device synchronisation) and the allocation is retried once.
 #include <map>
 #include <mutex>
 #include <unordered_map>
 #include <utility>
+#include <vector>
 
 #include "common.h"
 
@@ -24,8 +29,15 @@ struct Key {
   }
 };
 
+// A cached block and the events (recorded on other streams at its free)
+// that the next user's stream must wait for.
+struct Block {
+  void* p;
+  std::vector<hipEvent_t> waits;
+};
+
 std::mutex g_mu;
-std::map<Key, std::multimap<size_t, void*>> g_free;
+std::map<Key, std::multimap<size_t, Block>> g_free;
 std::unordered_map<void*, std::pair<int, size_t>> g_size;  // every cached or live block
 
 size_t round_size(size_t n) {
@@ -42,8 +54,9 @@ void release_device_locked(int dev) {
       continue;
     }
     for (auto& kv : it->second) {
-      (void)hipFree(kv.second);
-      g_size.erase(kv.second);
+      for (hipEvent_t e : kv.second.waits) (void)hipEventDestroy(e);
+      (void)hipFree(kv.second.p);
+      g_size.erase(kv.second.p);
     }
     it = g_free.erase(it);
   }
@@ -80,8 +93,21 @@ int dev_alloc(void** out, size_t bytes, hipStream_t s) {
   auto& fl = g_free[Key{dev, s}];
   auto it = fl.lower_bound(want);
   if (it != fl.end() && it->first <= 2 * want) {
-    *out = it->second;
+    Block blk = std::move(it->second);
     fl.erase(it);
+    int rc = ZP_OK;
+    for (hipEvent_t e : blk.waits) {
+      if (rc == ZP_OK && hipStreamWaitEvent(s, e, 0) != hipSuccess)
+        rc = fail(ZP_ERR_DEVICE, "stream wait on a freed block's event failed");
+      (void)hipEventDestroy(e);
+    }
+    if (rc != ZP_OK) {  // keep the block out of circulation: its users may still run
+      g_size.erase(blk.p);
+      (void)hipDeviceSynchronize();
+      (void)hipFree(blk.p);
+      return rc;
+    }
+    *out = blk.p;
     return ZP_OK;
   }
   void* p = nullptr;
@@ -98,13 +124,31 @@ int dev_alloc(void** out, size_t bytes, hipStream_t s) {
   return ZP_OK;
 }
 
-void dev_free(void* p, hipStream_t s) {
+void dev_free(void* p, hipStream_t s) { dev_free_after(p, s, nullptr, 0); }
+
+void dev_free_after(void* p, hipStream_t s, const hipStream_t* also, size_t n_also) {
   if (!p) return;
   if (t_arena && !t_arena->measuring) return;  // arena memory is released with the arena
+  Block blk{p, {}};
+  for (size_t i = 0; i < n_also; i++) {
+    if (also[i] == s) continue;
+    hipEvent_t e = nullptr;
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess ||
+        hipEventRecord(e, also[i]) != hipSuccess) {
+      (void)hipGetLastError();
+      if (e) (void)hipEventDestroy(e);
+      (void)hipStreamSynchronize(also[i]);  // no event: wait for that stream here instead
+      continue;
+    }
+    blk.waits.push_back(e);
+  }
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_size.find(p);
-  if (it == g_size.end()) return;
-  g_free[Key{it->second.first, s}].emplace(it->second.second, p);
+  if (it == g_size.end()) {
+    for (hipEvent_t e : blk.waits) (void)hipEventDestroy(e);
+    return;
+  }
+  g_free[Key{it->second.first, s}].emplace(it->second.second, std::move(blk));
 }
 
 void dev_release_cache() {

@@ -1,4 +1,5 @@
 // api.hip -- the C ABI declared in include/zipgpu.h.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -120,6 +121,39 @@ int check_types(const zp_code* c, int eval_type, uint32_t D) {
       return fail(ZP_ERR_SHAPE, "Cannot fit " + std::to_string(w) +
                                     "-bit wide codeword entries in 32 bits entries");
   }
+  return ZP_OK;
+}
+
+// validate_input's bit budget of the linear combinations
+// (zip-plus/src/pcs/utils.rs:24-54): with d = log2(row_len),
+//   2*ceil_log2(max(REP*d, 1)) + ceil_log2(max(d, 1)) + challenge_bits
+//     + (COEFF_BIT_WIDTH - 1) + ceil_log2(batch)  <=  CombR::NUM_BITS,
+// where Chal = i128 (every ZipTypes of the reference: protocol/src/lib.rs:
+// 475-593, zip-plus/benches) and challenge_bits doubles when Comb has a
+// degree bound above zero (alphas are drawn too), i.e. for BinaryPoly<D>
+// and DensePolynomial<i64, D> with D > 1.  COEFF_BIT_WIDTH is 1 for binary
+// polys (binary_u64.rs:330), 32 for i32, 64 for i64 and DensePolynomial<i64,
+// D> (dense.rs:431).  Returns ZP_ERR_SHAPE where the reference's assert
+// fires, with its message.
+int lc_budget_check(uint32_t rep, uint64_t row_len, int eval_type, uint32_t D, uint32_t L,
+                    uint64_t batch) {
+  auto clog2 = [](uint64_t x) {
+    uint32_t r = 0;
+    while ((1ull << r) < x) r++;
+    return (uint64_t)r;
+  };
+  if (row_len == 0) return fail(ZP_ERR_SHAPE, "row length must be positive");
+  const uint64_t d = (uint64_t)(63 - __builtin_clzll(row_len));  // usize::ilog2
+  const uint64_t cw_bits = clog2(std::max<uint64_t>((uint64_t)rep * d, 1));
+  const bool poly = eval_type == ZP_EVAL_BINPOLY || eval_type == ZP_EVAL_DENSE_I64;
+  const uint64_t chal_bits = (poly && D > 1) ? 256 : 128;
+  const uint64_t coeff_bits = eval_type == ZP_EVAL_BINPOLY ? 1 : eval_type == ZP_EVAL_I32 ? 32 : 64;
+  const uint64_t lc = 2 * cw_bits + clog2(std::max<uint64_t>(d, 1)) + chal_bits +
+                      (coeff_bits - 1) + clog2(batch);
+  if (lc > 64ull * L)
+    return fail(ZP_ERR_SHAPE, "The number of bits used for linear combinations is too large: " +
+                                  std::to_string(lc) + " bits, max allowed is " +
+                                  std::to_string(64 * L) + " bits");
   return ZP_OK;
 }
 
@@ -358,6 +392,18 @@ int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_c
 }
 
 void zp_code_destroy(zp_code* code) { free_code(code); }
+
+int zp_validate_input(uint32_t rep, uint64_t row_len, int eval_type, uint32_t D,
+                      uint32_t comb_limbs, uint32_t batch) {
+  if (batch == 0) return fail(ZP_ERR_SHAPE, "Batch must contain at least one polynomial");
+  if (comb_limbs < 1 || comb_limbs > 8)
+    return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 1..8");
+  if (eval_type < ZP_EVAL_I32 || eval_type > ZP_EVAL_DENSE_I64)
+    return fail(ZP_ERR_INVALID_PARAM, "unknown eval type");
+  if ((eval_type == ZP_EVAL_BINPOLY || eval_type == ZP_EVAL_DENSE_I64) && (D == 0 || D > 64))
+    return fail(ZP_ERR_INVALID_PARAM, "degree bound must be within 1..64");
+  return lc_budget_check(rep, row_len, eval_type, D, comb_limbs, batch);
+}
 uint64_t zp_code_row_len(const zp_code* c) { return c ? c->row_len : 0; }
 uint64_t zp_code_codeword_len(const zp_code* c) { return c ? c->N : 0; }
 uint32_t zp_code_depth(const zp_code* c) { return c ? c->depth : 0; }
@@ -501,10 +547,20 @@ int zp_hint_open_column(const zp_hint* h, uint64_t col, void* h_out, zp_stream s
   return ZP_OK;
 }
 
+int zp_hint_record_stream(zp_hint* h, zp_stream stream) {
+  if (!h) return fail(ZP_ERR_INVALID_PARAM, "null hint");
+  hipStream_t s = (hipStream_t)stream;
+  if (s == h->stream) return ZP_OK;
+  for (hipStream_t u : h->used_on)
+    if (u == s) return ZP_OK;
+  h->used_on.push_back(s);
+  return ZP_OK;
+}
+
 void zp_hint_free(zp_hint* h) {
   if (!h || !h->owned) return;
-  dev_free(h->d_cw, h->stream);
-  dev_free(h->d_layers, h->stream);
+  dev_free_after(h->d_cw, h->stream, h->used_on.data(), h->used_on.size());
+  dev_free_after(h->d_layers, h->stream, h->used_on.data(), h->used_on.size());
   delete h;
 }
 

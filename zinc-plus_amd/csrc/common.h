@@ -6,6 +6,7 @@
 
 #include <atomic>
 #include <string>
+#include <vector>
 
 #include "../../include/zipgpu.h"
 
@@ -74,6 +75,9 @@ struct zp_hint {
   uint8_t* d_layers = nullptr;   // (2N-1) x 32 bytes, leaves first, root last
   uint64_t N = 0, num_rows = 0, elem = 0, per_poly = 0;
   uint32_t batch = 0, height = 0;
+  // Other streams that used the buffers (zp_hint_record_stream): the free
+  // keeps the blocks from the next user until their work has finished.
+  std::vector<hipStream_t> used_on;
 };
 
 namespace zpi {
@@ -90,6 +94,10 @@ struct Arena {
 void set_arena(Arena* a);
 int dev_alloc(void** out, size_t bytes, hipStream_t s);
 void dev_free(void* p, hipStream_t s);
+// dev_free of a block that was also used on the n_also streams `also`: the
+// next allocation that takes it first waits for the work those streams held
+// at this call.
+void dev_free_after(void* p, hipStream_t s, const hipStream_t* also, size_t n_also);
 void dev_release_cache();
 uint8_t* pinned_scratch(size_t bytes);
 
@@ -154,6 +162,8 @@ int launch_gather_openings(const zp_hint* h, const uint64_t* d_cols, uint32_t n_
                            void* d_columns, void* d_paths, hipStream_t s);
 inline bool pow2(uint64_t x) { return x && !(x & (x - 1)); }
 int check_types(const zp_code* c, int eval_type, uint32_t D);
+int lc_budget_check(uint32_t rep, uint64_t row_len, int eval_type, uint32_t D, uint32_t L,
+                    uint64_t batch);
 uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D);
 uint64_t eval_bytes_of(int eval_type, uint32_t D);
 int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,

@@ -12,9 +12,12 @@
 // fixed, so a workgroup that owns a few r values and all 8^depth chunks of a
 // row runs the whole transform on chip and writes each output once.
 //
-// Exactness (no overflow at any supported size): |B| < 2^23 (i32), stage 0
-// < 2^41, stage 1 < 2^59 (i64) for binary inputs; i32 inputs need i64 base
-// sums (< 2^54) and i128 stages.
+// Exactness: for binary inputs |B| < 2^23 (i32), stage 0 < 2^41 and stage 1
+// < 2^59 (i64), so depths up to 2 (every new_with_optimal_depth code) never
+// overflow.  An explicit depth-3 code can pass 2^63 at stage 2: there the
+// i64 arithmetic wraps, exactly as the reference's unchecked build does (its
+// checked build panics instead).  i32 inputs need i64 base sums (< 2^54)
+// and i128 stages, which cannot overflow.
 #include "common.h"
 
 namespace zpi {

@@ -304,18 +304,9 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
     return fail(ZP_ERR_SHAPE, "hint does not match the code, rows, batch or eval type");
   // validate_input (pcs/utils.rs:16-78): the combined row's bit budget, and
   // the point has one coordinate per variable of the polys.
+  ZP_TRY(lc_budget_check(code->rep, row_len, eval_type, D, L, batch));
   {
-    const bool binpoly = eval_type == ZP_EVAL_BINPOLY;
     const uint64_t d = (uint64_t)__builtin_ctzll(row_len);
-    const uint64_t cw_bits = ceil_log2(std::max<uint64_t>(code->rep * d, 1));
-    const uint64_t chal_bits = (binpoly && D > 1) ? 256 : 128;
-    const uint64_t coeff_bits = binpoly ? 1 : 32;
-    const uint64_t lc = 2 * cw_bits + ceil_log2(std::max<uint64_t>(d, 1)) + chal_bits +
-                        (coeff_bits - 1) + ceil_log2(batch);
-    if (lc > 64ull * L)
-      return fail(ZP_ERR_SHAPE, "The number of bits used for linear combinations is too large: " +
-                                    std::to_string(lc) + " bits, max allowed is " +
-                                    std::to_string(64 * L) + " bits");
     const uint64_t nv = __builtin_ctzll(num_rows) + d;
     if (num_vars != nv)
       return fail(ZP_ERR_INVALID_PARAM, "Invalid point (expect point to have " +
@@ -386,7 +377,11 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   if (!binpoly) bufs.add((void**)&d_wide, rows * row_len * 8);
   ZP_TRY(bufs.alloc(s));
   int rc = ZP_OK;
+  // Every exit goes through here.  On a failure, copies from the per-thread
+  // pinned buffer may still be in flight: they must finish before the
+  // buffer is reused or regrown by the next call on this thread.
   auto done = [&](int r) {
+    if (r != ZP_OK) (void)hipStreamSynchronize(s);
     dev_free(bufs.base, s);
     return r;
   };
@@ -511,7 +506,7 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
     ts_put(t, o_open + off, std::min(kPiece, open_total - off));
   }
   if (rc == ZP_OK && npieces == 0) ts_put(t, o_comb, comb_bytes);
-  if (rc != ZP_OK) (void)hipStreamSynchronize(s);  // no copy may still target `pin`
+  if (rc != ZP_OK) (void)hipStreamSynchronize(s);  // before the events go
   for (hipEvent_t e : pev)
     if (e) (void)hipEventDestroy(e);
   return done(rc);

@@ -36,7 +36,7 @@ __all__ = [
     "ZipTypes", "BinPolyTypes", "IntTypes", "I64Types", "DensePolyTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
     "merkle_root", "lib", "set_option", "BatchCommit", "CommitServer", "PcsProverTranscript",
-    "step0_commit",
+    "step0_commit", "validate_input",
 ]
 
 
@@ -90,6 +90,15 @@ def set_option(name, value):
 def _check(rc):
     if rc != L.ZP_OK:
         raise _ERRORS.get(rc, ZipError)(rc, L.last_error())
+
+
+def validate_input(rep, row_len, types, batch_size):
+    """validate_input's bit budget (zip-plus/src/pcs/utils.rs:24-54), the
+    assert ZipPlus::commit runs first (phase_commit.rs:70-77): raises
+    ShapeError where the reference panics.  Host only (zp_validate_input)."""
+    _check(L.load().zp_validate_input(int(rep), int(row_len), types.code,
+                                      types.degree_plus_one, types.comb_limbs,
+                                      int(batch_size)))
 
 
 def _stream(stream=None):
@@ -369,6 +378,15 @@ class ZipPlusHint:
                                                 _stream(stream)))
         return out
 
+    def device_codewords_ptr(self):
+        """The device address of the codewords (zp_hint_codewords)."""
+        return int(self._lib.zp_hint_codewords(self._h) or 0)
+
+    def record_stream(self, stream):
+        """zp_hint_record_stream: work on `stream` reads the hint, so its
+        memory is not handed out again until that work has finished."""
+        _check(self._lib.zp_hint_record_stream(self._h, ctypes.c_void_p(stream.cuda_stream)))
+
     def layer(self, level, stream=None):
         n = self.num_leaves >> level
         out = np.empty((n, 32), dtype=np.uint8)
@@ -572,6 +590,8 @@ class ZipPlus:
     def commit(self, pp, polys, stream=None):
         """ZipPlus::commit (phase_commit.rs:61-101) -> (hint, commitment)."""
         t = self._stack(pp, polys)
+        validate_input(pp.linear_code.REPETITION_FACTOR, pp.linear_code.row_len(), self.types,
+                       t.shape[0])
         h = ctypes.c_void_p()
         root = (ctypes.c_uint8 * 32)()
         _check(L.load().zp_commit(pp.linear_code._h, self.types.code, self.types.degree_plus_one,

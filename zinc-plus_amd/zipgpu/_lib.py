@@ -87,6 +87,8 @@ SIGNATURES = {
     "zp_merkle_prove": (_int, [_c_void_p, _u64, _c_void_p, ctypes.POINTER(_u32), _c_void_p]),
     "zp_hint_open_column": (_int, [_c_void_p, _u64, _c_void_p, _c_void_p]),
     "zp_hint_free": (None, [_c_void_p]),
+    "zp_hint_record_stream": (_int, [_c_void_p, _c_void_p]),
+    "zp_validate_input": (_int, [_u32, _u64, _int, _u32, _u32, _u32]),
     "zp_batch_create": (_int, [_c_void_p, _u32, _u32, _u32, ctypes.POINTER(_c_void_p)]),
     "zp_batch_commit": (_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p]),
     "zp_batch_hint": (_c_void_p, [_c_void_p, _u32]),
