@@ -162,6 +162,24 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t degree_plus_one, cons
               uint64_t num_rows, uint32_t batch, zp_hint** hint_out, uint8_t h_root[32],
               zp_stream stream);
 
+/* zp_commit without any allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device or pinned memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The same commit as zp_commit for `batch` BinaryPoly<degree_plus_one>
+ * polys (eval_type must be ZP_EVAL_BINPOLY), written into caller buffers:
+ * d_codewords (batch x num_rows x codeword_len elements, as
+ * zp_hint_codewords) and d_layers ((2 codeword_len - 1) x 32 bytes, leaves
+ * first; the root is the last 32 bytes).  Fully asynchronous on `stream`;
+ * no host synchronisation.  Transient scratch comes from one static device
+ * array of 64 MiB per device (held until the process ends); a commit that
+ * needs more (bit-plane words of the IPRS matrix-core encoder plus 1.5 x
+ * 32 bytes per chunk and column of multi-chunk leaves) returns ZP_ERR_OOM
+ * before any work.  Every call on a device shares that array: calls that
+ * overlap on one device race and must be serialised by the caller. */
+int zp_commit_noalloc(const zp_code* code, int eval_type, uint32_t degree_plus_one,
+                      const void* d_evals, uint64_t num_rows, uint32_t batch, void* d_codewords,
+                      uint8_t* d_layers, zp_stream stream);
+
 /* Hint accessors (the reference keeps these in host memory). */
 int zp_hint_info(const zp_hint* hint, uint64_t* cw_bytes_per_poly, uint32_t* batch,
                  uint64_t* num_leaves, uint32_t* height);

@@ -157,6 +157,10 @@ int lc_budget_check(uint32_t rep, uint64_t row_len, int eval_type, uint32_t D, u
   return ZP_OK;
 }
 
+bool iprs_uses_mfma(const zp_code* c, uint32_t D) {
+  return iprs_mfma_eligible(c, D) && pick(g_force_iprs, c->tune_iprs, 0) == 0;
+}
+
 // Encodes with the variant in force for this code (types already checked).
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
                         uint64_t num_rows, void* out, hipStream_t s) {
@@ -169,7 +173,7 @@ int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void*
                                     (const int64_t*)evals, num_rows, out, s);
   if (c->kind == ZP_CODE_IPRS) {
     if (eval_type == ZP_EVAL_BINPOLY) {
-      if (iprs_mfma_eligible(c, D) && pick(g_force_iprs, c->tune_iprs, 0) == 0)
+      if (iprs_uses_mfma(c, D))
         return launch_iprs_binary_mfma(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
       return launch_iprs_binary(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
     }
@@ -281,7 +285,7 @@ extern "C" {
 
 int zp_init(int device) {
   ZP_HIP(hipSetDevice(device));
-  return ZP_OK;
+  return noalloc_init_device();
 }
 
 const char* zp_status_string(int status) {

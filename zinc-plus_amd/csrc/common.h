@@ -131,6 +131,9 @@ void iprs_afrag(uint32_t bd, uint32_t bl, uint32_t ksteps, const int32_t* ftab, 
 int launch_iprs_binary(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
                        int64_t* out, hipStream_t s);
 bool iprs_mfma_eligible(const zp_code* c, uint32_t D);
+// Bytes of bit-plane words the matrix-core encoder takes from the cache for
+// `rows` rows (one allocation per launch).
+uint64_t iprs_mfma_scratch_bytes(const zp_code* c, uint64_t rows, uint32_t D);
 int launch_iprs_binary_mfma(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
                             int64_t* out, hipStream_t s);
 int launch_iprs_i32(const zp_code* c, const int32_t* evals, uint64_t rows, void* out,
@@ -151,6 +154,12 @@ int commit_enqueue(const zp_code* code, int eval_type, uint32_t D, const void* d
                    bool allow_overlap);
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
                         uint64_t num_rows, void* out, hipStream_t s);
+// The variant encode_rows_variant runs for an IPRS binary-poly code: true
+// for the matrix-core encoder.
+bool iprs_uses_mfma(const zp_code* c, uint32_t D);
+// noalloc.hip: resolves the device address of the static scratch array of
+// the current device (once per device).
+int noalloc_init_device();
 // wide.hip: LinearCode::encode_wide over Int<L>, L in 1..8.
 int launch_encode_wide(const zp_code* c, uint32_t L, const uint64_t* in, uint64_t rows,
                        uint64_t* out, hipStream_t s);

@@ -449,6 +449,11 @@ int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t 
 
 }  // namespace
 
+uint64_t iprs_mfma_scratch_bytes(const zp_code* cd, uint64_t rows, uint32_t D) {
+  const uint64_t NC = cd->depth == 1 ? 8 : 64;
+  return rows * NC * 4 * cd->mfma_ksteps * ((D + 15) / 16 * 16) * sizeof(uint32_t);
+}
+
 bool iprs_mfma_eligible(const zp_code* cd, uint32_t D) {
   return cd->d_afrag != nullptr && D >= 1 && D <= 64 && (cd->depth == 1 || cd->depth == 2) &&
          cd->row_len < (1ull << 31);
