@@ -24,7 +24,7 @@ namespace zpi {
 // Library-wide overrides of the kernel variants; -1 leaves the choice to
 // the code's tuned variant (zp_autotune) or the default.
 int g_force_iprs = -1, g_force_raa = -1, g_force_overlap = -1, g_force_wide = -1,
-    g_force_tree = -1, g_force_ctree = -1;
+    g_force_tree = -1, g_force_ctree = -1, g_force_comb = -1;
 
 
 uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D) {
@@ -315,7 +315,8 @@ int zp_set_option(const char* name, int64_t value) {
                       {"commit_overlap", &g_force_overlap, 1},
                       {"wide_base", &g_force_wide, 1},
                       {"merkle_tree", &g_force_tree, 1},
-                      {"chunk_tree", &g_force_ctree, 1}};
+                      {"chunk_tree", &g_force_ctree, 1},
+                      {"combined_row", &g_force_comb, 1}};
   for (const auto& o : opts) {
     if (std::strcmp(name, o.name) != 0) continue;
     if (value < -1 || value > o.max)

@@ -117,7 +117,8 @@ int grant_lds(size_t lds) {
 
 // Library-wide overrides of the kernel variants (zp_set_option): -1 = use
 // the code's tuned or default choice.
-extern int g_force_iprs, g_force_raa, g_force_overlap, g_force_wide, g_force_tree, g_force_ctree;
+extern int g_force_iprs, g_force_raa, g_force_overlap, g_force_wide, g_force_tree, g_force_ctree,
+    g_force_comb;
 
 // Host-side tables (params.cpp).
 int iprs_params(uint64_t row_len, uint64_t depth, uint64_t rep, uint32_t* bl, uint32_t* bd);
