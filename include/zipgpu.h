@@ -94,6 +94,8 @@ const char* zp_last_error(void);
  *                   of two) folded in registers by up to 256 lanes and
  *                   merged by 4-lane quads (default), 1 = the whole tree
  *                   in LDS, or one launch per level, for every chunk count.
+ *   "combined_row": 0 = the combined row's slice sums on the int8 matrix
+ *                   cores (default, up to 2^24 rows), 1 = 64-bit VALU sums.
  * Returns ZP_ERR_INVALID_PARAM for an unknown name or value. */
 int zp_set_option(const char* name, int64_t value);
 

@@ -82,7 +82,8 @@ def set_option(name, value):
     matrix cores where the code has the int8 tables, 1 per-element), "merkle_tree"
     (0 a 4-lane quad per parent node, 1 a thread per parent), "chunk_tree" (0
     register folds and quad merges for 4 to 2048 chunks per column, 1 the LDS
-    tree for every count).
+    tree for every count), "combined_row" (0 slice sums on the int8 matrix
+    cores, 1 64-bit VALU sums).
     """
     _check(L.load().zp_set_option(name.encode(), int(value)))
 
