@@ -11,7 +11,9 @@ Also reported on the same line:
   roofline      the dominant kernel (the matrix-core IPRS encoder) timed with
                 HIP events on the stream it runs on; traffic from the
                 committed rocprofv3 counter summary when present
-  cpu_baseline  the CPU oracle committing a bounded sample of the workload
+  cpu_baseline  the reference's CPU commit of the same workload, restated on
+                its parallel axes with its AVX-512 paths (all allotted host
+                threads, and one thread)
   workloads     plain Zip 2^16 i32 commit; RAA 2^22 BinaryPoly<64> encode,
                 commit and combined row; the sign-flipping RAA code's encode
                 at the same shape; 64 mixed commits as one batch plan
@@ -59,30 +61,38 @@ def traffic_per_launch():
         return None
 
 
-def cpu_baseline(seconds=12.0):
-    """The oracle (CPU restatement of the reference) committing a bounded
-    sample of the workload: 8 rows of the same code (2^17 evals)."""
+def cpu_baseline(gpu_root, seconds=10.0):
+    """The reference's CPU commit of the whole workload, restated
+    (oracle/baseline.cpp): 2^20 BinaryPoly<32> under the same IPRS code, 64
+    rows, on the reference's parallel axes (codeword indices, blocks, columns,
+    pairs) with its AVX-512 widening and 16-lane BLAKE3 where the host has
+    AVX-512; all allotted host threads (up to 16), then one thread.  Its root
+    must equal the GPU's."""
     import numpy as np
 
     from oracle import oracle
     from zipgpu import synth
 
     threads = min(16, os.cpu_count() or 1)
-    oracle.set_threads(threads)
-    rows = 8
-    n = rows * ROW_LEN
-    x = synth.binary_polys(2, n, D).view(np.uint64)
-    oracle.commit("iprs", x, rows, ROW_LEN, D=D)  # warm
+    x = synth.binary_polys(2, N_EVALS, D).view(np.uint64)  # the GPU step's input
+    rows = N_EVALS // ROW_LEN
+    root = oracle.baseline_commit_iprs(x, rows, ROW_LEN, D=D, threads=threads)["root"]
     reps, t0 = 0, time.perf_counter()
     while True:
-        oracle.commit("iprs", x, rows, ROW_LEN, D=D)
+        oracle.baseline_commit_iprs(x, rows, ROW_LEN, D=D, threads=threads)
         reps += 1
         el = time.perf_counter() - t0
-        if el >= seconds or reps >= 50:
+        if el >= seconds / 2 or reps >= 10:
             break
-    return {"value": n * reps / el, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"oracle commit of {rows} rows x 2^14 evals (2^17 BinaryPoly<32>, same "
-                      f"IPRS code, BLAKE3 Merkle over 8-row columns), {reps} reps in {el:.1f} s"}
+    t1 = time.perf_counter()
+    oracle.baseline_commit_iprs(x, rows, ROW_LEN, D=D, threads=1)
+    one = time.perf_counter() - t1
+    simd = "AVX-512" if oracle.baseline_has_avx512() else "portable (no AVX-512)"
+    return {"value": N_EVALS * reps / el, "unit": "evals/s", "cores": threads, "kind": "port",
+            "one_thread_value": N_EVALS / one, "root_equals_gpu": root == gpu_root,
+            "sample": f"the full workload: commit of 2^20 BinaryPoly<32>, IPRS rows 2^14 x 64, "
+                      f"BLAKE3 Merkle over 65536 columns; {reps} commits in {el:.2f} s on "
+                      f"{threads} threads ({simd}), one commit in {one:.2f} s on 1 thread"}
 
 
 def wall_ms(torch, fn, steps, warmup=1):
@@ -351,19 +361,22 @@ def serve(zipgpu, reqs, slots, arrivals, open_delays, prove=False):
     """Run the requests through a CommitServer: commit i arrives at
     arrivals[i] (seconds from the start), its opening (or, with prove, its
     full ZipPlus::prove at the point q["point"]) open_delays[i] after its
-    root is back.  -> (wall seconds, commit latencies us, open latencies us)."""
+    root is back.  -> (wall seconds, commit latencies us, open latencies us,
+    HBM peak, queue depths: jobs waiting for a slot or memory, sampled at
+    each arrival before it is submitted)."""
     import heapq
     srv = zipgpu.CommitServer(slots=slots)
     t0 = time.perf_counter()
     due = [(a, 0, i) for i, a in enumerate(arrivals)]  # (time, kind 0 commit / 1 open, index)
     heapq.heapify(due)
-    ids, lat_c, lat_o, left = {}, [], [], 2 * len(reqs)
+    ids, lat_c, lat_o, depth, left = {}, [], [], [], 2 * len(reqs)
     while left:
         now = time.perf_counter() - t0
         while due and due[0][0] <= now:
             _, kind, i = heapq.heappop(due)
             q = reqs[i]
             if kind == 0:
+                depth.append(srv.stats()["pending"])
                 ids[srv.commit(q["pp"], q["types"], q["x"])] = i
             elif prove:
                 srv.prove(q["rid"], q["point"])
@@ -387,50 +400,70 @@ def serve(zipgpu, reqs, slots, arrivals, open_delays, prove=False):
     wall = time.perf_counter() - t0
     peak = srv.stats()["hbm_peak"]
     srv.close()
-    return wall, lat_c, lat_o, peak
+    return wall, lat_c, lat_o, peak, depth
+
+
+def poisson_arrivals(synth, seed, rate, count=None, window=None):
+    """Arrival times of a Poisson process of `rate` per second: the first
+    `count`, or all within `window` seconds (times from 0)."""
+    import numpy as np
+    n = count if count is not None else int(rate * window * 1.5) + 16
+    u = (synth.splitmix64(seed, n) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    gaps = -np.log1p(-u) / rate
+    t = np.cumsum(gaps) - gaps[0]
+    return t[:count] if count is not None else t[t < window]
 
 
 def leg_served(torch, zipgpu, synth, dev):
-    """Commit+open service: Poisson arrivals, log2 n uniform in 14..22, D 32/64,
-    IPRS/RAA; openings (combined row + 147 columns with Merkle paths) a
-    seeded delay after each commit."""
+    """Commit+open service: Poisson arrivals over a fixed window, log2 n
+    uniform in 14..22, D 32/64, IPRS/RAA; openings (combined row + 147
+    columns with Merkle paths) a seeded delay after each commit."""
     import numpy as np
-    count, rate = 32, 300.0  # requests, arrivals per second
-    reqs, _ = served_requests(zipgpu, synth, 5, count, 14, 22)
+    pool_n, rate, window = 48, 400.0, 1.5  # distinct requests, arrivals per second, seconds
+    pool, _ = served_requests(zipgpu, synth, 5, pool_n, 14, 22)
+    arrivals = poisson_arrivals(synth, 77, rate, window=window)
+    count = len(arrivals)
+    reqs = [dict(pool[i % pool_n]) for i in range(count)]
     total = sum(q["n"] for q in reqs)
-    u = (synth.splitmix64(77, 2 * count) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
-    gaps = -np.log1p(-u[:count]) / rate
-    arrivals = np.cumsum(gaps) - gaps[0]
-    delays = -np.log1p(-u[count:]) * 1e-3  # openings: mean 1 ms after the root
-    serve(zipgpu, reqs[:4], 4, [0.0] * 4, [0.0] * 4)  # warm the codes and the device cache
-    wall, lc, lo, peak = serve(zipgpu, reqs, 4, arrivals, delays)
-    roots = [q["root"] for q in reqs]
-    # Saturated throughput with 4 slots and with 1: medians of 3 alternating
-    # runs (one run is ~0.1 s, short enough for host noise to show).
+    u = (synth.splitmix64(78, count) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    delays = -np.log1p(-u) * 1e-3  # openings: mean 1 ms after the root
+    serve(zipgpu, [dict(q) for q in pool[:4]], 4, [0.0] * 4, [0.0] * 4)  # warm codes and cache
+    wall, lc, lo, peak, depth = serve(zipgpu, reqs, 4, arrivals, delays)
+    # Roots of requests that share inputs agree.
+    same = all(reqs[i]["root"] == reqs[i % pool_n]["root"] for i in range(count))
+    # Saturated throughput with 4 slots and with 1 over the 48 distinct
+    # requests: medians of 3 alternating runs.
+    sat = [dict(q) for q in pool]
     s4, s1 = [], []
     for _ in range(3):
-        s4.append(serve(zipgpu, reqs, 4, [0.0] * count, [0.0] * count)[0])
-        s1.append(serve(zipgpu, reqs, 1, [0.0] * count, [0.0] * count)[0])
+        s4.append(serve(zipgpu, sat, 4, [0.0] * pool_n, [0.0] * pool_n)[0])
+        s1.append(serve(zipgpu, sat, 1, [0.0] * pool_n, [0.0] * pool_n)[0])
+    sat_total = sum(q["n"] for q in pool)
     sat4, sat1 = float(np.median(s4)), float(np.median(s1))
-    same = roots == [q["root"] for q in reqs]
-    # The same arrivals with full proofs (Fiat-Shamir transcript from each
-    # root, b, combined row, 147 openings, proof bytes) instead of openings.
-    pwall, _, lp, _ = serve(zipgpu, reqs, 4, arrivals, delays, prove=True)
+    # Full proofs (Fiat-Shamir transcript from each root, b, combined row,
+    # 147 openings, proof bytes) instead of openings: 32 requests, Poisson 300/s.
+    pr = [dict(q) for q in pool[:32]]
+    parr = poisson_arrivals(synth, 79, 300.0, count=32)
+    pwall, _, lp, _, _ = serve(zipgpu, pr, 4, parr, delays[:32], prove=True)
     pct = lambda a, p: float(np.percentile(np.asarray(a), p))  # noqa: E731
-    return {"workload": f"{count} commit+open requests, Poisson {rate:.0f}/s, log2 n ~ U{{14..22}}, "
-                        "D 32/64, IPRS/RAA, host inputs; openings: combined row Int<5> + 147 "
-                        "columns with Merkle paths, exp(1 ms) after each root; 4 slots; the same arrivals "
-                        "again with full proofs (prove_*: transcript from the root, 147 openings)",
-            "total_evals": total, "wall_s": wall, "evals_per_s": total / wall,
+    return {"workload": f"{count} commit+open requests (a pool of {pool_n} distinct inputs), "
+                        f"Poisson {rate:.0f}/s over {window} s, log2 n ~ U{{14..22}}, D 32/64, "
+                        "IPRS/RAA, host inputs; openings: combined row Int<5> + 147 columns with "
+                        "Merkle paths, exp(1 ms) after each root; 4 slots; queue depth = jobs "
+                        "waiting for a slot or memory at each arrival; prove_*: 32 requests "
+                        "at 300/s with full proofs (transcript from the root, 147 openings)",
+            "requests": count, "total_evals": total, "wall_s": wall, "evals_per_s": total / wall,
             "commit_latency_us_p50": pct(lc, 50), "commit_latency_us_p99": pct(lc, 99),
             "open_latency_us_p50": pct(lo, 50), "open_latency_us_p99": pct(lo, 99),
+            "queue_depth_mean": float(np.mean(depth)), "queue_depth_p99": pct(depth, 99),
+            "queue_depth_max": int(max(depth)),
             "hbm_peak_bytes": peak,
             "prove_wall_s": pwall, "prove_latency_us_p50": pct(lp, 50),
             "prove_latency_us_p99": pct(lp, 99),
-            "saturated_4_slots_evals_per_s": total / sat4,
-            "one_at_a_time_evals_per_s": total / sat1,
+            "saturated_4_slots_evals_per_s": sat_total / sat4,
+            "one_at_a_time_evals_per_s": sat_total / sat1,
             "served_vs_one_at_a_time": sat1 / sat4,
-            "roots_equal_across_runs": same}
+            "roots_equal_for_equal_inputs": same}
 
 
 def copy_calibration():
@@ -568,7 +601,7 @@ def main():
         line["workloads"] = legs
     if not args.no_cpu:
         log("cpu baseline ...")
-        line["cpu_baseline"] = cpu_baseline()
+        line["cpu_baseline"] = cpu_baseline(c.root)
         log(f"cpu: {line['cpu_baseline']['value']:.4g} evals/s")
     print(json.dumps(line), flush=True)
 

@@ -222,6 +222,29 @@ def commit(kind, evals, num_rows, row_len, batch=1, depth=-1, D=32, eval_type="b
     return {"root": root.tobytes(), "cw": cw, "layers": layers}
 
 
+def baseline_commit_iprs(evals, num_rows, row_len, D=32, threads=1, simd=True, depth=-1, rep=4,
+                         want_cw=False):
+    """The timed CPU baseline (oracle/baseline.cpp): ZipPlus::commit of one
+    BinaryPoly<D> poly under IPRS on the reference's parallel axes, with its
+    AVX-512 widening and 16-lane BLAKE3 where the CPU has AVX-512.
+    -> dict(root, cw)."""
+    ev = np.ascontiguousarray(evals, dtype=np.uint64)
+    N = row_len * rep
+    cw = np.zeros(num_rows * N * D, dtype=np.int64) if want_cw else None
+    root = np.zeros(32, dtype=np.uint8)
+    rc = load().or_baseline_commit_iprs_binary(_u64(row_len), _i64(depth), _u64(rep), _u64(D),
+                                               _p(ev), _u64(num_rows), ctypes.c_int(threads),
+                                               ctypes.c_int(1 if simd else 0),
+                                               _p(cw) if want_cw else None, _p(root))
+    if rc:
+        raise ValueError(f"baseline rc {rc}")
+    return {"root": root.tobytes(), "cw": cw}
+
+
+def baseline_has_avx512():
+    return bool(load().or_baseline_has_avx512())
+
+
 def combined_row(evals, alphas, coeffs, num_rows, row_len, batch=1, D=64, L=5):
     """-> uint64 [row_len, L] limbs of Int<L>."""
     ev = np.ascontiguousarray(evals, dtype=np.uint64)
