@@ -322,6 +322,12 @@ def leg_batched(torch, zipgpu, synth, dev):
     roots = [c.root for c in plan.commit(xs)]
     plan_ms = wall_ms(torch, lambda: plan.commit(xs), 5, 1)
     tuned_plain_ms = wall_ms(torch, plain, 3, 1)
+    # The same graph on one branch stream: every item's device work back to
+    # back with no host work between them (the serial device time).
+    serial = zipgpu.BatchCommit(items, graph=True, autotune=False, n_streams=1)
+    serial.commit(xs)
+    serial_ms = wall_ms(torch, lambda: serial.commit(xs), 5, 1)
+    del serial
     # The plan's roots equal the plain path's.
     same = roots == [zipgpu.ZipPlus(t).commit(pp, [x])[1].root
                      for (pp, t), x in zip(items, xs)]
@@ -331,6 +337,7 @@ def leg_batched(torch, zipgpu, synth, dev):
                         "RAA (row isqrt(n) rounded up to 2^k), seeded mix 4",
             "total_evals": total, "plain_ms": plain_ms, "plain_evals_per_s": total / plain_ms * 1e3,
             "plain_autotuned_ms": tuned_plain_ms, "batched_graph_ms": plan_ms,
+            "serial_graph_1_stream_ms": serial_ms, "batched_vs_serial": serial_ms / plan_ms,
             "batched_evals_per_s": total / plan_ms * 1e3, "batched_vs_plain": plain_ms / plan_ms,
             "roots_equal_plain": same, "hint_bytes": info["hint_bytes"],
             "scratch_bytes": info["scratch_bytes"]}

@@ -248,15 +248,18 @@ int zp_batch_info(const zp_batch* plan, uint64_t* hint_bytes, uint64_t* scratch_
 void zp_batch_destroy(zp_batch* plan);
 
 /* ---- Autotuner (no reference counterpart) --------------------------------
- * Times each kernel variant of `code` on synthetic inputs of the given
- * shape (encoder for IPRS binary polys; RAA kernel; commit overlap) and
- * stores the fastest in the code; later encodes and commits use it.  Must
- * not run concurrently with other uses of the code. */
+ * Times the kernel variants and tile parameters of `code` on synthetic
+ * inputs of the given shape and stores the fastest in the code; later
+ * encodes and commits use them: the RAA kernel (full row, 8- or 4-plane
+ * groups), the chunk-tree fold of the leaf hashing (1, 2, 4 or 8 chunk
+ * chaining values per lane before the quad merges), and the encode/hash
+ * overlap.  IPRS binary codes take the matrix-core encoder wherever it
+ * applies.  Must not run concurrently with other uses of the code. */
 int zp_autotune(zp_code* code, int eval_type, uint32_t degree_plus_one, uint64_t num_rows,
                 uint32_t batch, zp_stream stream);
 /* The tuned variants (-1 = not tuned), numbered as in zp_set_option. */
 int zp_code_tuning(const zp_code* code, int32_t* iprs_encoder, int32_t* raa_kernel,
-                   int32_t* commit_overlap);
+                   int32_t* commit_overlap, int32_t* chunk_tree_fold);
 
 /* ---- Commit/open service ------------------------------------------------
  * Continuous batching on one device (no reference counterpart; it serves

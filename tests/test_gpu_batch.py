@@ -163,13 +163,15 @@ def test_autotune_keeps_results(zg, dev, oracle, kind, row_len, D, rows):
     say("autotune", kind, row_len, D, rows)
     code = make_code(zg, kind, row_len)
     types = zg.BinPolyTypes(D)
-    assert code.tuning() == {"iprs_encoder": -1, "raa_kernel": -1, "commit_overlap": -1}
+    assert code.tuning() == {"iprs_encoder": -1, "raa_kernel": -1, "commit_overlap": -1,
+                             "chunk_tree_fold": -1}
     t = code.autotune(types, rows)
     if kind == "iprs":
-        assert t["iprs_encoder"] in (0, 1) and t["raa_kernel"] == -1
+        assert t["iprs_encoder"] == 0 and t["raa_kernel"] == -1  # matrix cores where eligible
     else:
         assert t["raa_kernel"] in (0, 1, 2) and t["iprs_encoder"] == -1
     assert t["commit_overlap"] in (0, 1)
+    assert t["chunk_tree_fold"] in (-1, 1, 2, 4, 8)
     n = rows * row_len
     x = synth.binary_polys(900 + row_len, n, D)
     xd = to_dev(x, dev)

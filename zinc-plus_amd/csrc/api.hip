@@ -206,7 +206,8 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
     for (uint32_t b = 0; b < batch; b++)
       ZP_TRY(encode_rows_variant(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride,
                                  num_rows, h->d_cw + b * h->per_poly, s));
-    return launch_leaf_hash(h->d_cw, rows_all, N, (uint32_t)elem, h->d_layers, s);
+    return launch_leaf_hash(h->d_cw, rows_all, N, (uint32_t)elem, h->d_layers, s,
+                            code->tune_ctree_m);
   }
   hipStream_t s2;
   ZP_TRY(aux_stream(&s2));
@@ -234,7 +235,7 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
   if (hipEventRecord(ev[kGroups], s2) != hipSuccess ||
       hipStreamWaitEvent(s, ev[kGroups], 0) != hipSuccess)
     rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "stream join failed") : rc;
-  if (rc == ZP_OK) rc = launch_cv_merge(cv[0], cv[1], nchunks, N, h->d_layers, s);
+  if (rc == ZP_OK) rc = launch_cv_merge(cv[0], cv[1], nchunks, N, h->d_layers, s, code->tune_ctree_m);
   for (auto e : ev)
     if (e) (void)hipEventDestroy(e);
   dev_free(cv[0], s);

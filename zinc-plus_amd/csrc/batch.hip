@@ -186,9 +186,9 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
   if (eval_type == ZP_EVAL_I64 || eval_type == ZP_EVAL_DENSE_I64) {
     // One encoder (the Int<L> kernels): nothing to tune.
   } else if (code->kind == ZP_CODE_IPRS) {
-    std::vector<int> c = {1};
-    if (eval_type == ZP_EVAL_BINPOLY && iprs_mfma_eligible(code, D)) c = {0, 1};
-    tune(&code->tune_iprs, c, encode_all);
+    // The matrix-core encoder wherever it applies (the VALU encoder is 14x
+    // slower at every such shape); otherwise the VALU encoder.
+    code->tune_iprs = (eval_type == ZP_EVAL_BINPOLY && iprs_mfma_eligible(code, D)) ? 0 : 1;
   } else {
     std::vector<int> c;
     if (raa_full_row_eligible(code, D)) c.push_back(0);
@@ -199,6 +199,11 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
   auto commit_all = [&] {
     return commit_enqueue(code, eval_type, D, evals, num_rows, batch, h, s, true);
   };
+  // The chunk-tree fold of the leaf hashing (chunks per lane), then the
+  // encode/hash overlap with that fold.
+  const std::vector<int> folds = ctree_fold_candidates(
+      leaf_chunks(num_rows * batch, (uint32_t)cw_elem_bytes_of(code, eval_type, D)));
+  tune(&code->tune_ctree_m, folds, commit_all);
   tune(&code->tune_overlap, {0, 1}, commit_all);
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
@@ -208,11 +213,12 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
 }
 
 int zp_code_tuning(const zp_code* code, int32_t* iprs_encoder, int32_t* raa_kernel,
-                   int32_t* commit_overlap) {
+                   int32_t* commit_overlap, int32_t* chunk_tree_fold) {
   if (!code) return fail(ZP_ERR_INVALID_PARAM, "null code");
   if (iprs_encoder) *iprs_encoder = code->tune_iprs;
   if (raa_kernel) *raa_kernel = code->tune_raa;
   if (commit_overlap) *commit_overlap = code->tune_overlap;
+  if (chunk_tree_fold) *chunk_tree_fold = code->tune_ctree_m;
   return ZP_OK;
 }
 

@@ -64,6 +64,9 @@ struct zp_code {
   mutable int tune_iprs = -1;     // 0 matrix-core encoder, 1 VALU encoder
   mutable int tune_raa = -1;      // 0 full-row kernel, 1 plane groups (8), 2 plane groups (4)
   mutable int tune_overlap = -1;  // commit: 1 overlap encode and hash on two streams, 0 not
+  // Chunks folded per lane by the chunk-tree kernel of this code's commits
+  // (1, 2, 4 or 8; -1 = the shape heuristic).
+  mutable int tune_ctree_m = -1;
 };
 
 // A commitment hint kept on the device (ZipPlusHint).
@@ -176,14 +179,19 @@ int lc_budget_check(uint32_t rep, uint64_t row_len, int eval_type, uint32_t D, u
                     uint64_t batch);
 uint64_t cw_elem_bytes_of(const zp_code* c, int eval_type, uint32_t D);
 uint64_t eval_bytes_of(int eval_type, uint32_t D);
+// ctree_m: chunks folded per lane in the chunk trees (1, 2, 4, 8), or -1
+// for the shape heuristic.
 int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
-                     uint8_t* leaves, hipStream_t s);
+                     uint8_t* leaves, hipStream_t s, int ctree_m = -1);
 uint64_t leaf_chunks(uint64_t nrows, uint32_t elem);
 int launch_chunk_cvs(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
                      uint64_t ci_begin, uint64_t ci_end, uint32_t* cvs, uint8_t* leaves,
                      hipStream_t s);
 int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
-                    uint8_t* leaves, hipStream_t s);
+                    uint8_t* leaves, hipStream_t s, int ctree_m = -1);
+// The chunk-tree folds the quad kernel can take for a shape (empty when it
+// does not apply): the tuner's candidates.
+std::vector<int> ctree_fold_candidates(uint64_t nchunks);
 // Chaining values (8 words each) of n full chunks of the stream that frames
 // `entry_bytes`-byte entries of `data` as 0x6 || entry || 0x7, starting at
 // stream offset q0 with chunk counter c0.

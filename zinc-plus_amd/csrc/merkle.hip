@@ -575,16 +575,30 @@ int launch_cv_quad_tree(const uint32_t* cvs, uint64_t width, uint32_t C, uint8_t
 // from 4 to 2048 take the quad kernel (C lanes per column, M chunks each);
 // others the LDS tree kernel, or one launch per level when a column's tree
 // does not fit in LDS.
+std::vector<int> ctree_fold_candidates(uint64_t nchunks) {
+  std::vector<int> c;
+  if (!(pow2(nchunks) && nchunks >= 4 && nchunks <= 2048)) return c;
+  // M chunks per lane; C = nchunks / M lanes per column, a multiple of 4
+  // (the quads) and at most 256 (one workgroup).
+  for (int M = 1; M <= 8; M *= 2)
+    if ((uint64_t)M * 4 <= nchunks && nchunks / M <= 256) c.push_back(M);
+  return c;
+}
+
 int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
-                    uint8_t* leaves, hipStream_t s) {
+                    uint8_t* leaves, hipStream_t s, int ctree_m) {
   if (nchunks <= 1) return ZP_OK;
   if (g_force_ctree != 1 && pow2(nchunks) && nchunks >= 4 && nchunks <= 2048) {
     // Up to 8 chunks per lane, fewer while that leaves under 2^18 lanes in
     // all (4 waves per SIMD) and a column stays within 256 lanes.  More
     // lanes per column add quad levels: at 16 chunks x 2^16 columns, 4 lanes
-    // of 4 chunks take 26.5 us and 8 lanes of 2 take 31.4 us.
+    // of 4 chunks take 26.5 us and 8 lanes of 2 take 31.4 us.  A tuned fold
+    // (ctree_m) replaces the rule when the shape admits it.
     uint64_t M = std::min<uint64_t>(8, nchunks / 4);
     while (M > 1 && width * (nchunks / M) < (1u << 18) && nchunks / M < 256) M /= 2;
+    if (ctree_m > 0)
+      for (int c : ctree_fold_candidates(nchunks))
+        if (c == ctree_m) M = (uint64_t)c;
     const uint32_t C = (uint32_t)(nchunks / M);
     switch (M) {
       case 1: return launch_cv_quad_tree<1>(cvs, width, C, leaves, s);
@@ -616,7 +630,7 @@ int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t wid
 }
 
 int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32_t elem,
-                     uint8_t* leaves, hipStream_t s) {
+                     uint8_t* leaves, hipStream_t s, int ctree_m) {
   if (width == 0) return ZP_OK;
   const uint64_t nchunks = leaf_chunks(nrows, elem);
   uint32_t* cv[2] = {nullptr, nullptr};
@@ -629,7 +643,7 @@ int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32
     }
   }
   int rc = launch_chunk_cvs(rows, nrows, width, elem, 0, nchunks, cv[0], leaves, s);
-  if (rc == ZP_OK) rc = launch_cv_merge(cv[0], cv[1], nchunks, width, leaves, s);
+  if (rc == ZP_OK) rc = launch_cv_merge(cv[0], cv[1], nchunks, width, leaves, s, ctree_m);
   dev_free(cv[0], s);
   dev_free(cv[1], s);
   return rc;

@@ -109,7 +109,8 @@ extern "C" int zp_commit_noalloc(const zp_code* code, int eval_type, uint32_t D,
   arena.size = kScratchBytes;
   set_arena(&arena);
   int rc = encode_rows_variant(code, eval_type, D, d_evals, rows_all, d_codewords, s);
-  if (rc == ZP_OK) rc = launch_leaf_hash(h.d_cw, rows_all, N, (uint32_t)elem, d_layers, s);
+  if (rc == ZP_OK)
+    rc = launch_leaf_hash(h.d_cw, rows_all, N, (uint32_t)elem, d_layers, s, code->tune_ctree_m);
   if (rc == ZP_OK) rc = launch_merkle_layers(d_layers, N, s);
   set_arena(nullptr);
   return rc;

@@ -257,10 +257,10 @@ class LinearCode:
 
     def tuning(self):
         """The tuned variants (-1 = not tuned), numbered as in set_option."""
-        v = [ctypes.c_int32() for _ in range(3)]
+        v = [ctypes.c_int32() for _ in range(4)]
         _check(self._lib.zp_code_tuning(self._h, *[ctypes.byref(x) for x in v]))
         return {"iprs_encoder": v[0].value, "raa_kernel": v[1].value,
-                "commit_overlap": v[2].value}
+                "commit_overlap": v[2].value, "chunk_tree_fold": v[3].value}
 
     def encode(self, row, types):
         """LinearCode::encode of one row."""

@@ -99,7 +99,7 @@ SIGNATURES = {
     "zp_batch_destroy": (None, [_c_void_p]),
     "zp_autotune": (_int, [_c_void_p, _int, _u32, _u64, _u32, _c_void_p]),
     "zp_code_tuning": (_int, [_c_void_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
-                              ctypes.POINTER(_i32)]),
+                              ctypes.POINTER(_i32), ctypes.POINTER(_i32)]),
     "zp_server_create": (_int, [_c_void_p, ctypes.POINTER(_c_void_p)]),
     "zp_server_commit": (_int, [_c_void_p, _c_void_p, _c_void_p, _c_void_p,
                                 ctypes.POINTER(_u64)]),
