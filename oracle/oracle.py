@@ -12,7 +12,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# ZP_ORACLE_LIB names another build of the oracle (an instrumented one).
+LIB_PATH = os.environ.get("ZP_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _vp = ctypes.c_void_p

@@ -632,7 +632,7 @@ static void b3_compress(const uint32_t cv[8], const uint32_t block[16], uint64_t
 
 static void load_block(const uint8_t* p, uint64_t len, uint32_t m[16]) {
   uint8_t b[64] = {0};
-  std::memcpy(b, p, len);
+  if (len) std::memcpy(b, p, len);  // p may be null for the empty input
   for (int i = 0; i < 16; i++)
     m[i] = (uint32_t)b[4 * i] | ((uint32_t)b[4 * i + 1] << 8) | ((uint32_t)b[4 * i + 2] << 16) |
            ((uint32_t)b[4 * i + 3] << 24);

@@ -8,7 +8,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(os.path.dirname(_HERE), "libzipgpu.so")
+# ZIPGPU_LIB names another build of the same library (an instrumented one).
+LIB_PATH = os.environ.get("ZIPGPU_LIB") or os.path.join(os.path.dirname(_HERE), "libzipgpu.so")
 
 ZP_OK = 0
 ZP_ERR_INVALID_PARAM = 1
