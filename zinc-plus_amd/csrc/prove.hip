@@ -102,6 +102,157 @@ plane_sums_kernel(const Word* __restrict__ evals, const Fp* __restrict__ q1,
   for (int k = 0; k < 9; k++) o[k] = s[k];
 }
 
+// ---- Plane sums on the matrix cores ------------------------------------
+//
+// P_c[j] = sum_col q1[col] bit_c(x_j[col]) is, for one row j, the product
+// (37 x row_len seven-bit slices of q1) . (row_len x D bit matrix of the row):
+// q1 < 2^256 is cut into 37 unsigned slices (7 x 37 = 259 bits), the A
+// operand of v_mfma_i32_16x16x64_i8 (three 16-row tiles, rows 37..47 zero),
+// shared by every row; the B operand of one K-step is 64 consecutive
+// columns x 16 planes of the row.  Lane (g, n) of a wave loads the 16 evals
+// of its lane group (columns 16g .. 16g + 15 of the K-step) and
+// byte-transposes the byte of its plane (v_perm); plane c' of the byte
+// enters as bits scaled by 2^max(c' - 1, 0) <= 64, shifted back in the
+// epilogue.  Every i32 partial is below 127 * 64 * 64 per K-step, and a
+// block holds at most kMaxBlockSteps K-steps, so the sums are exact.
+typedef int v4i __attribute__((ext_vector_type(4)));
+constexpr uint32_t kMaxBlockSteps = 2048;  // 2048 * 520192 < 2^31
+
+// frag[(kt * 3 + mt) * 64 + lane]: slice 16 mt + lane % 16 of q1[col] for
+// col = 64 kt + 16 (lane / 16) + e in byte e.
+__global__ void __launch_bounds__(256)
+q1_afrag_kernel(const Fp* __restrict__ q1, v4i* __restrict__ frag, uint64_t row_len,
+                uint32_t ksteps) {
+  const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= ksteps * 192) return;
+  const uint32_t lane = idx % 64, mt = (idx / 64) % 3, kt = idx / 192;
+  const uint32_t m = 16 * mt + lane % 16;
+  uint32_t w[4] = {0, 0, 0, 0};
+  if (m < 37) {
+    const uint32_t b0 = 7 * m, wi = b0 / 64, sh = b0 % 64;
+#pragma unroll
+    for (uint32_t e = 0; e < 16; e++) {
+      const uint64_t col = (uint64_t)kt * 64 + 16 * (lane / 16) + e;
+      if (col >= row_len) continue;
+      const uint64_t* q = q1[col].w;
+      uint64_t v = q[wi] >> sh;
+      if (sh > 57 && wi < 3) v |= q[wi + 1] << (64 - sh);
+      w[e / 4] |= (uint32_t)(v & 127) << (8 * (e % 4));
+    }
+  }
+  frag[idx] = v4i{(int)w[0], (int)w[1], (int)w[2], (int)w[3]};
+}
+
+// Grid: (row, plane chunk q of 16, column block).  The 4 waves of a
+// workgroup take every fourth K-step of the block; their slice sums are
+// added in LDS and wave 0 recombines them into the nine 32-bit words of
+// part[(row * nblk + blk) * D + c].
+template <typename Word>
+__global__ void __launch_bounds__(256)
+plane_sums_mfma_kernel(const Word* __restrict__ evals, const v4i* __restrict__ frag,
+                       uint32_t* __restrict__ part, uint64_t row_len, uint32_t ksteps,
+                       uint32_t spb, uint32_t nblk, uint32_t D) {
+  __shared__ int32_t red[4][64][12];
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t g = lane >> 4, n = lane & 15;
+  const uint32_t Q = (D + 15) / 16;
+  uint64_t rest = blockIdx.x;
+  const uint32_t blk = (uint32_t)(rest % nblk);
+  rest /= nblk;
+  const uint32_t q = (uint32_t)(rest % Q);
+  const uint64_t row = rest / Q;
+  const uint32_t c = 16 * q + n;               // this lane's plane (output column)
+  const uint32_t beta = c / 8, cp = c % 8;     // its byte and bit within the byte
+  const uint32_t h = beta / 4, k = beta % 4;   // 32-bit half of the eval, byte in it
+  const uint32_t sel = ((4 + k) << 8) | k;     // bytes k of the two operands
+  const uint32_t mk = 0x01010101u << (cp == 0 ? 0 : cp - 1);
+  const Word* x = evals + row * row_len;
+  const uint32_t kt0 = blk * spb, kt1 = min(ksteps, kt0 + spb);
+  v4i acc[3] = {v4i{0, 0, 0, 0}, v4i{0, 0, 0, 0}, v4i{0, 0, 0, 0}};
+  for (uint32_t kt = kt0 + wave; kt < kt1; kt += 4) {
+    const uint64_t col0 = (uint64_t)kt * 64 + 16 * g;
+    uint32_t e[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const uint64_t col = col0 + i;
+      const Word v = col < row_len ? x[col] : (Word)0;
+      e[i] = (uint32_t)((uint64_t)v >> (32 * h));
+    }
+    v4i bf;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const uint32_t u = __builtin_amdgcn_perm(e[4 * v + 1], e[4 * v + 0], sel);
+      const uint32_t w = __builtin_amdgcn_perm(e[4 * v + 3], e[4 * v + 2], sel);
+      const uint32_t z = __builtin_amdgcn_perm(w, u, 0x05040100u);
+      bf[v] = (int)((cp == 0 ? z : z >> 1) & mk);
+    }
+#pragma unroll
+    for (int mt = 0; mt < 3; mt++)
+      acc[mt] = __builtin_amdgcn_mfma_i32_16x16x64_i8(frag[(kt * 3 + mt) * 64 + lane], bf,
+                                                      acc[mt], 0, 0, 0);
+  }
+#pragma unroll
+  for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) red[wave][lane][4 * mt + v] = acc[mt][v];
+  __syncthreads();
+  if (wave != 0) return;
+  // Lane (g, n) holds slices 16 mt + 4 g + v of plane c (scaled); the sum
+  // is below 2^256 * 2^log2(row_len) < 2^288, kept in five 64-bit limbs.
+  const int sc = cp == 0 ? 0 : (int)cp - 1;
+  uint64_t V[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+  for (int mt = 0; mt < 3; mt++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const uint32_t m = 16 * mt + 4 * g + v;
+      if (m >= 37) continue;
+      const uint64_t t = (uint64_t)(uint32_t)(red[0][lane][4 * mt + v] + red[1][lane][4 * mt + v] +
+                                              red[2][lane][4 * mt + v] + red[3][lane][4 * mt + v]) >> sc;
+      const uint32_t b0 = 7 * m, wi = b0 / 64, sh = b0 % 64;
+      // V += t * 2^b0 (t < 2^31, so it spans at most two limbs).
+      uint64_t add[5] = {0, 0, 0, 0, 0};
+      add[wi] = t << sh;
+      if (wi + 1 < 5) add[wi + 1] = sh ? (t >> (64 - sh)) : 0;
+      uint64_t cy = 0;
+#pragma unroll
+      for (int i = 0; i < 5; i++) {
+        const uint64_t s1 = V[i] + add[i];
+        const uint64_t c1 = s1 < add[i];
+        const uint64_t s2 = s1 + cy;
+        cy = c1 | (s2 < cy);
+        V[i] = s2;
+      }
+    }
+#pragma unroll
+  for (int msk = 16; msk <= 32; msk *= 2) {
+    uint64_t o[5];
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)V[i], msk);
+      const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(V[i] >> 32), msk);
+      o[i] = ((uint64_t)hi << 32) | lo;
+    }
+    uint64_t cy = 0;
+#pragma unroll
+    for (int i = 0; i < 5; i++) {
+      const uint64_t s1 = V[i] + o[i];
+      const uint64_t c1 = s1 < o[i];
+      const uint64_t s2 = s1 + cy;
+      cy = c1 | (s2 < cy);
+      V[i] = s2;
+    }
+  }
+  if (g != 0 || c >= D) return;
+  uint32_t* o = part + (((uint64_t)row * nblk + blk) * D + c) * 9;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    o[2 * i] = (uint32_t)V[i];
+    o[2 * i + 1] = (uint32_t)(V[i] >> 32);
+  }
+  o[8] = (uint32_t)V[4];
+}
+
 // One thread per (poly row, plane): adds the blocks' partial sums and
 // reduces mod p.  The total is below row_len * p <= 2^red_bits * p: p * 2^k
 // is subtracted where it fits, k = red_bits-1 .. 0.
@@ -151,17 +302,32 @@ plane_reduce_kernel(const uint32_t* __restrict__ part, Fp* __restrict__ out, uin
   out[i] = r;
 }
 
-// b[j] = sum_b sum_c alpha_{b,c} * P[b][j][c] (all Montgomery form).
+// b[j] = sum_b sum_c alpha_{b,c} * P[b][j][c] (all Montgomery form).  One
+// wave per row: lane l takes the (poly, plane) pairs l, l + 64, ..., then
+// the wave adds its 64 partial sums in a butterfly.
 __global__ void __launch_bounds__(256)
 b_kernel(const Fp* __restrict__ planes, const Fp* __restrict__ alphas, Fp* __restrict__ b,
          uint64_t num_rows, uint32_t D, uint32_t batch, FpParams P) {
-  const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= num_rows) return;
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t j = (uint64_t)blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (j >= num_rows) return;  // whole waves leave together
   Fp acc = {{0, 0, 0, 0}};
-  for (uint32_t p = 0; p < batch; p++)
-    for (uint32_t c = 0; c < D; c++)
-      acc = fp_add(P, acc, fp_mul(P, alphas[p * D + c], planes[((uint64_t)p * num_rows + j) * D + c]));
-  b[j] = acc;
+  for (uint32_t t = lane; t < batch * D; t += 64) {
+    const uint32_t p = t / D, c = t % D;
+    acc = fp_add(P, acc, fp_mul(P, alphas[p * D + c], planes[((uint64_t)p * num_rows + j) * D + c]));
+  }
+#pragma unroll
+  for (int msk = 1; msk < 64; msk *= 2) {
+    Fp o;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const uint32_t lo = (uint32_t)__shfl_xor((int)(uint32_t)acc.w[i], msk);
+      const uint32_t hi = (uint32_t)__shfl_xor((int)(uint32_t)(acc.w[i] >> 32), msk);
+      o.w[i] = ((uint64_t)hi << 32) | lo;
+    }
+    acc = fp_add(P, acc, o);
+  }
+  if (lane == 0) b[j] = acc;
 }
 
 // i32 evals as u64 lanes (bit pattern zero-extended) for the combined row.
@@ -358,13 +524,27 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   bufs.add((void**)&d_q1, row_len * sizeof(Fp));
   bufs.add((void**)&d_q0, num_rows * sizeof(Fp));
   bufs.add((void**)&d_planes, rows * Dk * sizeof(Fp));
-  // Column blocks of the plane sums: at least 256 columns each, and enough
-  // blocks for ~2048 workgroups over all rows.
-  const uint32_t nblk = (uint32_t)std::max<uint64_t>(
-      1, std::min<uint64_t>(row_len / 256, (2048 + rows - 1) / rows));
-  const uint32_t cpb = (uint32_t)((row_len + nblk - 1) / nblk);
+  // Plane sums on the matrix cores (default): column blocks of whole
+  // K-steps (64 columns), enough for ~2048 workgroups over rows and plane
+  // chunks; or, with zp_set_option("combined_row", 1), the VALU kernel with
+  // blocks of at least 256 columns.
+  const bool mfma_sums = g_force_comb != 1;
+  const uint32_t ksteps = (uint32_t)((row_len + 63) / 64);
+  uint32_t nblk, cpb = 0, spb = 0;
+  if (mfma_sums) {
+    const uint64_t units = rows * ((Dk + 15) / 16);
+    nblk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(ksteps, (2048 + units - 1) / units));
+    spb = (ksteps + nblk - 1) / nblk;
+    if (spb > kMaxBlockSteps) spb = kMaxBlockSteps;
+    nblk = (ksteps + spb - 1) / spb;
+  } else {
+    nblk = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(row_len / 256, (2048 + rows - 1) / rows));
+    cpb = (uint32_t)((row_len + nblk - 1) / nblk);
+  }
   uint32_t* d_part;
   bufs.add((void**)&d_part, rows * nblk * Dk * 9 * sizeof(uint32_t));
+  v4i* d_qfrag = nullptr;
+  if (mfma_sums) bufs.add((void**)&d_qfrag, (uint64_t)ksteps * 192 * sizeof(v4i));
   bufs.add((void**)&d_b, num_rows * sizeof(Fp));
   bufs.add((void**)&d_alm, alm.size() * sizeof(Fp));
   bufs.add((void**)&d_al, al.size() * 8);
@@ -405,18 +585,31 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
                      d_rr + 2 * col_vars, row_vars, d_q0, P);
   const uint32_t dp_log2 = ceil_log2(Dk);
   const uint32_t red_bits = ceil_log2(row_len) + 1;
-  if (binpoly)
+  if (mfma_sums) {
+    hipLaunchKernelGGL(q1_afrag_kernel, dim3((ksteps * 192 + 255) / 256), dim3(256), 0, s, d_q1,
+                       d_qfrag, row_len, ksteps);
+    const uint32_t grid = (uint32_t)(rows * ((Dk + 15) / 16) * nblk);
+    if (binpoly)
+      hipLaunchKernelGGL(plane_sums_mfma_kernel<uint64_t>, dim3(grid), dim3(256), 0, s,
+                         (const uint64_t*)d_evals, (const v4i*)d_qfrag, d_part, row_len, ksteps,
+                         spb, nblk, Dk);
+    else
+      hipLaunchKernelGGL(plane_sums_mfma_kernel<uint32_t>, dim3(grid), dim3(256), 0, s,
+                         (const uint32_t*)d_evals, (const v4i*)d_qfrag, d_part, row_len, ksteps,
+                         spb, nblk, Dk);
+  } else if (binpoly) {
     hipLaunchKernelGGL(plane_sums_kernel<uint64_t>, dim3((uint32_t)(rows * nblk)),
                        dim3(64 * kSumWaves), 0, s, (const uint64_t*)d_evals, d_q1, d_part, row_len,
                        cpb, nblk, Dk, dp_log2);
-  else
+  } else {
     hipLaunchKernelGGL(plane_sums_kernel<uint32_t>, dim3((uint32_t)(rows * nblk)),
                        dim3(64 * kSumWaves), 0, s, (const uint32_t*)d_evals, d_q1, d_part, row_len,
                        cpb, nblk, Dk, dp_log2);
+  }
   hipLaunchKernelGGL(plane_reduce_kernel, dim3((uint32_t)((rows * Dk + 255) / 256)), dim3(256), 0,
                      s, d_part, d_planes, rows, nblk, Dk, red_bits, P);
-  hipLaunchKernelGGL(b_kernel, dim3((uint32_t)((num_rows + 255) / 256)), dim3(256), 0, s,
-                     d_planes, d_alm, d_b, num_rows, Dk, batch, P);
+  hipLaunchKernelGGL(b_kernel, dim3((uint32_t)((num_rows + 3) / 4)), dim3(256), 0, s, d_planes,
+                     d_alm, d_b, num_rows, Dk, batch, P);
   if (hipGetLastError() != hipSuccess) return done(fail(ZP_ERR_DEVICE, "launch failed"));
   uint8_t* dn = pin + up1;
   if (hipMemcpyAsync(dn, d_b, num_rows * sizeof(Fp), hipMemcpyDeviceToHost, s) != hipSuccess ||

@@ -114,6 +114,11 @@ uint64_t prove_scratch(const zp_commit_desc& d, uint32_t n_open, uint32_t limbs)
                d.num_rows * 16 + combined_row_scratch_bytes(Dk, c->row_len, d.batch, d.num_rows) +
                c->row_len * limbs * 8 + (uint64_t)n_open * (rows * elem + 24 + 64 * 32 + 8);
   if (d.eval_type != ZP_EVAL_BINPOLY) b += rows * c->row_len * 8;
+  // Plane-sum partials (at most ~2048 workgroups' worth beyond one block
+  // per row and plane chunk, 36 bytes per plane) and the q1 slice operand
+  // (3 KiB per 64 columns).
+  const uint64_t units = rows * ((Dk + 15) / 16);
+  b += (units + 2048) * 16 * 36 + (c->row_len + 63) / 64 * 3072;
   return b + (8u << 20);
 }
 
