@@ -22,14 +22,18 @@
  * it panics on an assert it returns ZP_ERR_SHAPE.  zp_last_error() gives the
  * message of the calling thread's last failure.
  *
- * Arithmetic follows the reference's unchecked build (the `unchecked`
- * feature of its benches, CHECK = false): codeword and combined-row values
- * are exact modulo their types, i.e. they wrap where a checked build would
- * panic.  The path's binary-poly, i32 and i64 codes cannot overflow at
- * depth <= 2 (every new_with_optimal_depth code); an explicit depth-3 IPRS
- * code over binary polys, and IPRS over DensePolynomial<i64, D> with
- * coefficients above about 2^40, can.  ZP_ERR_OVERFLOW is reserved for a
- * checked mode and not returned.
+ * By default arithmetic follows the reference's unchecked build (the
+ * `unchecked` feature of its benches, CHECK = false): codeword and
+ * combined-row values are exact modulo their types, i.e. they wrap where a
+ * checked build would panic.  A code marked with zp_code_set_checked is the
+ * checked build (IprsCode<.., CHECKED>, RaaConfig::CHECK_FOR_OVERFLOWS, as
+ * the protocol's codes are, protocol/src/lib.rs:574-583): its encodings
+ * return ZP_ERR_OVERFLOW exactly where the reference panics, and equal the
+ * unchecked ones everywhere else.  The path's binary-poly, i32 and i64
+ * codes cannot overflow at depth <= 2 (every new_with_optimal_depth code);
+ * an explicit depth-3 IPRS code over binary polys, IPRS over
+ * DensePolynomial<i64, D> with large coefficients, and encode_wide of large
+ * Int<L> rows can.
  *
  * Thread safety: codes are immutable after creation and may be shared by any
  * number of threads; hints are owned by one caller at a time.
@@ -109,6 +113,21 @@ int zp_set_option(const char* name, int64_t value);
  * rep is the const REP (REPETITION_FACTOR). */
 int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_code** out);
 void zp_code_destroy(zp_code* code);
+/* Marks a code checked (checked != 0) or unchecked (0, the default); the
+ * CHECK const of IprsCode<Zt, Config, REP, CHECK> (code/iprs.rs:20) and
+ * RaaConfig::CHECK_FOR_OVERFLOWS (code/raa.rs:14).  A checked code's
+ * zp_encode_rows, zp_encode_wide, zp_commit and served commits return
+ * ZP_ERR_OVERFLOW where the reference's mul_by_scalar::<true>(..).expect,
+ * add! or neg! panics (iprs.rs:76-84, pntt/radix8.rs:150-157,
+ * butterfly.rs:34-42, raa.rs:205-218, raa_sign_flip.rs:146-155); the
+ * output buffer's contents are then unspecified.  Inputs that provably
+ * cannot overflow (from the code's tables and, for integer rows, a scan of
+ * the inputs' magnitudes) take the ordinary kernels; others take checked
+ * kernels whose verdict the host reads, so such calls synchronize the
+ * stream and fail under stream capture.  Set it before the code is shared
+ * between threads. */
+int zp_code_set_checked(zp_code* code, int checked);
+int zp_code_is_checked(const zp_code* code);
 /* LinearCode::row_len / codeword_len; depth is Radix8PnttParams::depth. */
 uint64_t zp_code_row_len(const zp_code* code);
 uint64_t zp_code_codeword_len(const zp_code* code);
@@ -177,7 +196,9 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t degree_plus_one, cons
  * needs more (bit-plane words of the IPRS matrix-core encoder plus 1.5 x
  * 32 bytes per chunk and column of multi-chunk leaves) returns ZP_ERR_OOM
  * before any work.  Every call on a device shares that array: calls that
- * overlap on one device race and must be serialised by the caller. */
+ * overlap on one device race and must be serialised by the caller.  A
+ * checked IPRS code whose binary rows can overflow (depth 3) is refused
+ * with ZP_ERR_INVALID_PARAM: its check reads a verdict on the host. */
 int zp_commit_noalloc(const zp_code* code, int eval_type, uint32_t degree_plus_one,
                       const void* d_evals, uint64_t num_rows, uint32_t batch, void* d_codewords,
                       uint8_t* d_layers, zp_stream stream);

@@ -291,6 +291,30 @@ def raa_encode_wide(rows, row_len, L=5, rep=4, sign_flip=False):
     return out
 
 
+def iprs_encode_wide_checked(rows, row_len, depth=-1, L=5, rep=4):
+    """The checked build's IPRS encode_wide (IprsCode<.., CHECKED>): returns
+    (out, overflowed); overflowed is True where the reference panics, and
+    out then holds no meaningful values."""
+    x = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1, row_len, L)
+    out = np.zeros((x.shape[0], row_len * rep, L), dtype=np.uint64)
+    rc = load().or_iprs_encode_wide_checked(_u64(row_len), _i64(depth), _u64(rep), _u64(L),
+                                            _p(x), _u64(x.shape[0]), _p(out))
+    if rc not in (0, 2):
+        raise ValueError(f"oracle rc {rc}")
+    return out, rc == 2
+
+
+def raa_encode_wide_checked(rows, row_len, L=5, rep=4, sign_flip=False):
+    """The checked build's RAA encode_wide (CHECK_FOR_OVERFLOWS): (out, overflowed)."""
+    x = np.ascontiguousarray(rows, dtype=np.uint64).reshape(-1, row_len, L)
+    out = np.zeros((x.shape[0], row_len * rep, L), dtype=np.uint64)
+    rc = load().or_raa_encode_wide_checked(_u64(row_len), _u64(rep), ctypes.c_int(int(sign_flip)),
+                                           _u64(L), _p(x), _u64(x.shape[0]), _p(out))
+    if rc not in (0, 2):
+        raise ValueError(f"oracle rc {rc}")
+    return out, rc == 2
+
+
 def limbs_add(a, b):
     """Limb-wise a + b mod 2^(64L) over the last axis (uint64 limbs)."""
     a = np.asarray(a, dtype=np.uint64)

@@ -32,6 +32,7 @@ typedef unsigned __int128 u128;
 enum {
   OR_OK = 0,
   OR_ERR_INVALID_PARAM = 1,
+  OR_ERR_OVERFLOW = 2,
   OR_ERR_SHAPE = 3,
 };
 
@@ -865,7 +866,7 @@ extern "C" int or_commit(int kind, uint64_t row_len, int64_t depth, uint64_t rep
 // Int<L> (two's complement over L little-endian u64 limbs, mod 2^(64L)).
 // ---------------------------------------------------------------------------
 struct BigL {
-  uint64_t w[8];
+  uint64_t w[9];  // up to Int<8>, plus the guard limb of the checked folds
 };
 
 static void big_from_i128(BigL& a, i128 v, int L) {
@@ -887,7 +888,7 @@ static void big_add(BigL& a, const BigL& b, int L) {
 static void big_mul_i128(BigL& out, const BigL& a, i128 s, int L) {
   BigL b;
   big_from_i128(b, s, L);
-  uint64_t r[8] = {0};
+  uint64_t r[9] = {0};
   for (int i = 0; i < L; i++) {
     u128 carry = 0;
     for (int j = 0; i + j < L; j++) {
@@ -1075,6 +1076,154 @@ extern "C" int or_raa_encode_wide(uint64_t row_len, uint64_t rep, int sign_flip,
       big_store(dst + i * L, acc, l);
     }
   });
+  return OR_OK;
+}
+
+// ---------------------------------------------------------------------------
+// The checked build of encode_wide: IprsCode<.., CHECKED> folds with
+// mul_by_scalar::<true>(..).expect (code/iprs.rs:76-84: checked_mul of
+// Int<L> by the i64 entry) and add! (pntt/radix8.rs:150-157 over
+// col = 1..bl-1, butterfly.rs:34-42 over j = 1..7), and the checked RAA
+// accumulate add! (raa.rs:205-218) with neg! sign flips
+// (raa_sign_flip.rs:146-155); each panics at the first result outside
+// Int<L>.  Here every value is carried exactly in L + 1 limbs (a
+// sign-extended Int<L> times an entry below 2^31, and the sum of two
+// Int<L> values, both fit) and range-checked after each step; the first
+// failure returns OR_ERR_OVERFLOW (the panic).  Values that pass equal the
+// unchecked ones.
+// ---------------------------------------------------------------------------
+static BigL big_widen(const uint64_t* p, int L) {
+  BigL a;
+  for (int i = 0; i < L; i++) a.w[i] = p[i];
+  a.w[L] = (int64_t)p[L - 1] < 0 ? ~0ull : 0ull;
+  return a;
+}
+// An (L+1)-limb value lies in Int<L> iff its top limb is the sign
+// extension of limb L-1.
+static bool big_in_range(const BigL& a, int L) {
+  return a.w[L] == ((int64_t)a.w[L - 1] < 0 ? ~0ull : 0ull);
+}
+static bool big_is_min(const uint64_t* p, int L) {
+  for (int i = 0; i < L - 1; i++)
+    if (p[i]) return false;
+  return p[L - 1] == (1ull << 63);
+}
+
+static int pntt_wide_row_checked(const PnttParams& p, int L, const uint64_t* in, uint64_t* out) {
+  const uint64_t N = p.codeword_len, bd = p.base_dim, bl = p.base_len, d = p.depth;
+  const uint64_t shift = 3 * d;
+  const int W = L + 1;
+  for (uint64_t i = 0; i < N; i++) {
+    const uint64_t chunk = i >> p.base_dim_log2, row = i & (bd - 1);
+    const uint64_t o = octrev(chunk, d);
+    BigL acc = big_widen(in + o * L, L);  // from_ref
+    const int64_t* mrow = &p.base_matrix[row * bl];
+    for (uint64_t col = 1; col < bl; col++) {
+      BigL t;
+      big_mul_i128(t, big_widen(in + (o | (col << shift)) * L, L), (i128)mrow[col], W);
+      if (!big_in_range(t, L)) return OR_ERR_OVERFLOW;
+      big_add(acc, t, W);
+      if (!big_in_range(acc, L)) return OR_ERR_OVERFLOW;
+    }
+    big_store(out + i * L, acc, L);
+  }
+  for (uint64_t k = 0; k < d; k++) {
+    const uint64_t sub = bd << (3 * k);
+    const int64_t* tw = p.twiddles[k].data();
+    for (uint64_t blk = 0; blk < N; blk += 8 * sub) {
+      uint64_t* chunk = out + blk * L;
+      for (uint64_t i = 0; i < sub; i++) {
+        BigL xs[8];
+        for (uint64_t j = 0; j < 8; j++) xs[j] = big_widen(chunk + (j * sub + i) * L, L);
+        const int64_t* twi = tw + i * 56;
+        for (uint64_t r = 0; r < 8; r++) {
+          BigL a = xs[0];
+          for (uint64_t jm1 = 0; jm1 < 7; jm1++) {
+            BigL t;
+            big_mul_i128(t, xs[jm1 + 1], (i128)twi[jm1 * 8 + BUTTERFLY_TABLE[r][jm1]], W);
+            if (!big_in_range(t, L)) return OR_ERR_OVERFLOW;
+            big_add(a, t, W);
+            if (!big_in_range(a, L)) return OR_ERR_OVERFLOW;
+          }
+          big_store(chunk + (r * sub + i) * L, a, L);
+        }
+      }
+    }
+  }
+  return OR_OK;
+}
+
+extern "C" int or_iprs_encode_wide_checked(uint64_t row_len, int64_t depth, uint64_t rep,
+                                           uint64_t L, const uint64_t* in, uint64_t num_rows,
+                                           uint64_t* out) {
+  if (L < 1 || L > 8) return OR_ERR_INVALID_PARAM;
+  PnttParams p;
+  int rc = resolve_params(row_len, depth, rep, p);
+  if (rc) return rc;
+  std::vector<int> rcs(num_rows, OR_OK);
+  parallel_for(num_rows, [&](uint64_t r) {
+    rcs[r] = pntt_wide_row_checked(p, (int)L, in + r * row_len * L, out + r * p.codeword_len * L);
+  });
+  for (int v : rcs)
+    if (v) return v;
+  return OR_OK;
+}
+
+extern "C" int or_raa_encode_wide_checked(uint64_t row_len, uint64_t rep, int sign_flip,
+                                          uint64_t L, const uint64_t* in, uint64_t num_rows,
+                                          uint64_t* out) {
+  if (L < 1 || L > 8 || !is_pow2(rep) || !is_pow2(row_len)) return OR_ERR_INVALID_PARAM;
+  const uint64_t N = row_len * rep;
+  std::vector<uint64_t> p1(N), p2(N);
+  or_raa_perm(N, 1, p1.data());
+  or_raa_perm(N, 2, p2.data());
+  const int l = (int)L, W = l + 1;
+  std::vector<int> rcs(num_rows, OR_OK);
+  parallel_for(num_rows, [&](uint64_t r) {
+    const uint64_t* x = in + r * row_len * L;
+    uint64_t* dst = out + r * N * L;
+    std::vector<BigL> tmp(N), a(N);
+    for (uint64_t i = 0; i < N; i++) {  // repeat, flip_even_signs
+      const uint64_t* xi = x + (i % row_len) * L;
+      if (sign_flip && (i & 1) && big_is_min(xi, l)) {
+        rcs[r] = OR_ERR_OVERFLOW;
+        return;
+      }
+      tmp[i] = big_widen(xi, l);
+      if (sign_flip && (i & 1)) big_neg(tmp[i], W);
+    }
+    for (int pass = 0; pass < 2; pass++) {  // shuffle, accumulate (twice)
+      const std::vector<uint64_t>& perm = pass == 0 ? p1 : p2;
+      const std::vector<BigL>& src = pass == 0 ? tmp : a;
+      std::vector<BigL> res(N);
+      BigL acc = src[perm[0]];
+      res[0] = acc;
+      for (uint64_t i = 1; i < N; i++) {
+        big_add(acc, src[perm[i]], W);
+        if (!big_in_range(acc, l)) {
+          rcs[r] = OR_ERR_OVERFLOW;
+          return;
+        }
+        res[i] = acc;
+      }
+      if (pass == 0) {
+        for (uint64_t i = 0; i < N; i++) {
+          if (sign_flip && (i & 1)) {  // flip_even_signs after the first accumulate
+            if (big_is_min(res[i].w, l)) {
+              rcs[r] = OR_ERR_OVERFLOW;
+              return;
+            }
+            big_neg(res[i], W);
+          }
+          a[i] = res[i];
+        }
+      } else {
+        for (uint64_t i = 0; i < N; i++) big_store(dst + i * L, res[i], l);
+      }
+    }
+  });
+  for (int v : rcs)
+    if (v) return v;
   return OR_OK;
 }
 

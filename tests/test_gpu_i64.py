@@ -72,9 +72,10 @@ def _sizes_commit(oracle, kind, x, n, cols, types, eval_type):
 @pytest.mark.parametrize("empty_arb", [False, True])
 def test_step0_commit_matches_oracle(zg, dev, oracle, empty_arb):
     # ZincPlusPiop::step0_commit (protocol/src/prover.rs:218-262) with the
-    # protocol's IPRS set-up (one row per poly, row_len = poly size):
-    # binary, arbitrary and int witness traces committed, roots and public
-    # entries absorbed.
+    # protocol's IPRS set-up (checked codes, one row per poly, row_len = poly
+    # size, protocol/src/lib.rs:574-575): binary, arbitrary and int witness
+    # traces committed, roots and public entries absorbed.  The arbitrary
+    # trace's coefficients are small, as the checked codes require.
     say("step0", empty_arb)
     n = 1 << 10
     D = 32
@@ -82,7 +83,7 @@ def test_step0_commit_matches_oracle(zg, dev, oracle, empty_arb):
              ("i64", zg.I64Types())]
     params, wit, pub, host_w, host_p = [], [], [], [], []
     for g, (et, types) in enumerate(kinds):
-        code = zg.IprsCode.new_with_optimal_depth(n)
+        code = zg.IprsCode.new_with_optimal_depth(n, checked=True)
         pp = zg.ZipPlus(types).setup(n, code)
         params.append((pp, types))
         W = types.eval_words
@@ -90,7 +91,8 @@ def test_step0_commit_matches_oracle(zg, dev, oracle, empty_arb):
         if et == "binpoly":
             xs = [synth.binary_polys(200 + 10 * g + k, n, D) for k in range(nw + 1)]
         else:
-            xs = [evals_i64(200 + 10 * g + k, n * W, True) for k in range(nw + 1)]
+            xs = [evals_i64(200 + 10 * g + k, n * W, True) >> np.int64(16 if W > 1 else 0)
+                  for k in range(nw + 1)]
         host_w.append(xs[1:])
         host_p.append(xs[:1])
         wit.append([torch.from_numpy(v).to(dev) for v in xs[1:]])

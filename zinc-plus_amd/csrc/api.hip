@@ -1,5 +1,6 @@
 // api.hip -- the C ABI declared in include/zipgpu.h.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -164,6 +165,11 @@ bool iprs_uses_mfma(const zp_code* c, uint32_t D) {
 // Encodes with the variant in force for this code (types already checked).
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
                         uint64_t num_rows, void* out, hipStream_t s) {
+  if (c->checked) {
+    bool done = false;
+    ZP_TRY(encode_checked(c, eval_type, D, evals, num_rows, out, s, &done));
+    if (done) return ZP_OK;
+  }
   // i64 and DensePolynomial<i64, D> evals: the Int<L> encoders per
   // coefficient plane (wrapping in the Cw type, as the unchecked build).
   if (eval_type == ZP_EVAL_I64)
@@ -359,6 +365,26 @@ int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_c
         iprs_twiddles(bd, c->N, c->depth, k, tws[k].data());
         rc = upload(tws[k], &c->d_tw[k]);
       }
+      // Growth bound of the checked encodings: the base layer sums bl
+      // products with |f| <= max|f|; stage k multiplies by at most
+      // max over positions and outputs of 1 + sum_j |twiddle|.
+      int64_t fmax = 0;
+      for (int32_t v : f) fmax = std::max<int64_t>(fmax, v < 0 ? -(int64_t)v : v);
+      c->growth_log2 = std::log2((double)bl * (double)std::max<int64_t>(fmax, 1));
+      for (uint32_t k = 0; k < c->depth; k++) {
+        int64_t smax = 1;
+        const uint64_t npos = (uint64_t)bd << (3 * k);
+        for (uint64_t i = 0; i < npos; i++)
+          for (int r = 0; r < 8; r++) {
+            int64_t sum = 1;
+            for (int jm1 = 0; jm1 < 7; jm1++) {
+              const int64_t t = tws[k][i * 56 + jm1 * 8 + ((r * (jm1 + 1)) & 7)];
+              sum += t < 0 ? -t : t;
+            }
+            smax = std::max(smax, sum);
+          }
+        c->growth_log2 += std::log2((double)smax);
+      }
       // Tables of the matrix-core encoder (depth 1 or 2, K up to 256).
       const uint32_t ksteps = (bl + 63) / 64;
       if (rc == ZP_OK && (c->depth == 1 || c->depth == 2) && bd >= 16 && ksteps <= 4) {
@@ -380,6 +406,7 @@ int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_c
     else if (!pow2(row_len)) rc = fail(ZP_ERR_SHAPE, "Row length must be a power of two");
     else if (c->N > (1ull << 32)) rc = fail(ZP_ERR_INVALID_PARAM, "codeword too long");
     if (rc == ZP_OK) {
+      c->growth_log2 = 2 * std::log2((double)c->N);  // two accumulates of N terms
       std::vector<uint32_t> p1(c->N), p2(c->N);
       raa_perm(c->N, 1, p1.data());
       raa_perm(c->N, 2, p2.data());
@@ -410,6 +437,14 @@ int zp_validate_input(uint32_t rep, uint64_t row_len, int eval_type, uint32_t D,
     return fail(ZP_ERR_INVALID_PARAM, "degree bound must be within 1..64");
   return lc_budget_check(rep, row_len, eval_type, D, comb_limbs, batch);
 }
+int zp_code_set_checked(zp_code* c, int checked) {
+  if (!c) return fail(ZP_ERR_INVALID_PARAM, "null code");
+  c->checked = checked != 0;
+  return ZP_OK;
+}
+
+int zp_code_is_checked(const zp_code* c) { return c && c->checked ? 1 : 0; }
+
 uint64_t zp_code_row_len(const zp_code* c) { return c ? c->row_len : 0; }
 uint64_t zp_code_codeword_len(const zp_code* c) { return c ? c->N : 0; }
 uint32_t zp_code_depth(const zp_code* c) { return c ? c->depth : 0; }

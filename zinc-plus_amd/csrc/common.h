@@ -67,6 +67,14 @@ struct zp_code {
   // Chunks folded per lane by the chunk-tree kernel of this code's commits
   // (1, 2, 4 or 8; -1 = the shape heuristic).
   mutable int tune_ctree_m = -1;
+  // The CHECKED build of the code (IprsCode<.., CHECKED>, code/iprs.rs:20;
+  // RaaConfig::CHECK_FOR_OVERFLOWS, code/raa.rs:14): encodings whose exact
+  // values leave the Cw type fail with ZP_ERR_OVERFLOW instead of wrapping.
+  bool checked = false;
+  // log2 of how much larger than the largest input any intermediate value
+  // of an encoding can be (IPRS: bl * max|f| times, per stage, the largest
+  // 1 + sum_j |twiddle|; RAA: N^2 over the two accumulates).
+  double growth_log2 = 0;
 };
 
 // A commitment hint kept on the device (ZipPlusHint).
@@ -169,6 +177,13 @@ int launch_encode_wide(const zp_code* c, uint32_t L, const uint64_t* in, uint64_
                        uint64_t* out, hipStream_t s);
 int launch_encode_i64_planes(const zp_code* c, uint32_t D, uint32_t L, const int64_t* in,
                              uint64_t rows, void* out, hipStream_t s);
+// The checked encoding of a checked code's rows (eval_type as
+// encode_rows_variant): *done false means these inputs cannot overflow and
+// the ordinary encoder is to run; true means `out` holds the encoding.
+// ZP_ERR_OVERFLOW where the reference's checked build panics.
+int encode_checked(const zp_code* c, int eval_type, uint32_t D, const void* in, uint64_t rows,
+                   void* out, hipStream_t s, bool* done);
+inline double code_growth_log2(const zp_code* c) { return c->growth_log2; }
 // prove.hip: column values and Merkle paths of n_cols columns of a hint
 // (columns: n_cols x rows x elem bytes; paths: n_cols x (height-1) x 32).
 int launch_gather_openings(const zp_hint* h, const uint64_t* d_cols, uint32_t n_cols,

@@ -115,6 +115,66 @@ __device__ __forceinline__ void big_store(uint64_t* p, const Big<L>& a) {
   for (int i = 0; i < L; i++) p[i] = a.w[i];
 }
 
+// ---- Checked arithmetic (the CHECKED codes) -----------------------------
+//
+// The checked build folds with num_traits checked_mul / checked_add and
+// panics at the first result outside Int<L> = [-2^(64L-1), 2^(64L-1))
+// (code/iprs.rs:76-84, pntt/radix8.rs:150-157, butterfly.rs:34-42,
+// raa.rs:205-218, raa_sign_flip.rs:146-155).  These helpers compute the
+// same wrapping values as the unchecked ones and report whether the exact
+// result left that range.  Overflow bits reported to the host:
+constexpr uint32_t kOvfMul = 1, kOvfAdd = 2, kOvfNeg = 4;
+
+template <int L>
+__device__ __forceinline__ bool big_is_neg(const Big<L>& a) {
+  return (int64_t)a.w[L - 1] < 0;
+}
+
+// acc += x * m, with the product and the sum each checked.  |x| is formed
+// as an unsigned L-limb magnitude (Int::MIN maps to 2^(64L-1)), multiplied
+// by |m| into L limbs plus a high limb; the product fits iff the high limb
+// is zero and the magnitude is below 2^(64L-1), or equal to it for a
+// negative product.  The sum overflows iff both operands have one sign and
+// the result the other.
+template <int L>
+__device__ __forceinline__ uint32_t big_mac_checked(Big<L>& acc, const Big<L>& x, int32_t m) {
+  const bool xneg = big_is_neg(x);
+  Big<L> p = x;
+  if (xneg) big_neg(p);
+  const uint64_t um = (uint64_t)(m < 0 ? -(int64_t)m : (int64_t)m);
+  uint64_t hi = 0;
+#pragma unroll
+  for (int i = 0; i < L; i++) {
+    const uint64_t lo = p.w[i] * um;
+    const uint64_t s = lo + hi;
+    hi = __umul64hi(p.w[i], um) + (s < lo);
+    p.w[i] = s;
+  }
+  const bool neg = xneg != (m < 0);
+  uint32_t ovf = hi != 0 ? kOvfMul : 0;
+  if (big_is_neg(p)) {  // magnitude >= 2^(64L-1): only -2^(64L-1) fits
+    bool is_min = p.w[L - 1] == (1ull << 63);
+#pragma unroll
+    for (int i = 0; i < L - 1; i++) is_min = is_min && p.w[i] == 0;
+    if (!(neg && is_min)) ovf |= kOvfMul;
+  }
+  if (neg) big_neg(p);
+  const bool as = big_is_neg(acc), ps = big_is_neg(p);
+  big_add(acc, p);
+  if (as == ps && big_is_neg(acc) != as) ovf |= kOvfAdd;
+  return ovf;
+}
+
+// a = -a, checked: only Int::MIN has no negation.
+template <int L>
+__device__ __forceinline__ uint32_t big_neg_checked(Big<L>& a) {
+  bool is_min = a.w[L - 1] == (1ull << 63);
+#pragma unroll
+  for (int i = 0; i < L - 1; i++) is_min = is_min && a.w[i] == 0;
+  big_neg(a);
+  return is_min ? kOvfNeg : 0;
+}
+
 __device__ __forceinline__ uint32_t octrev_w(uint32_t c, uint32_t d) {
   uint32_t r = 0;
   for (uint32_t i = 0; i < d; i++) {
@@ -136,6 +196,11 @@ struct WideLayout {
   // Input elements are i32 (sign-extended; offsets in 4-byte units).  Only
   // the matrix-core base layer reads such inputs.
   bool in_i32 = false;
+  // Input elements are bits: sequence rp reads bit (rp % planes) of the
+  // u64 word at in_at(rp, k) (a bit-packed BinaryPoly whose coefficient
+  // planes are encoded one by one; in_plane is 0).  Only the per-element
+  // base kernel reads such inputs.
+  bool in_bits = false;
   __device__ __forceinline__ uint64_t in_at(uint64_t rp, uint64_t k) const {
     return (rp / planes) * in_row + (rp % planes) * in_plane + k * in_elem;
   }
@@ -146,13 +211,30 @@ struct WideLayout {
 
 constexpr uint32_t kWideTile = 256;
 
+// One input element of sequence rp at index k, as L limbs.
+template <int L>
+__device__ __forceinline__ Big<L> lay_load(const uint64_t* in, const WideLayout& lay,
+                                           uint64_t rp, uint64_t k) {
+  if (lay.in_bits) {
+    Big<L> a;
+    a.w[0] = (in[lay.in_at(rp, k)] >> (rp % lay.planes)) & 1u;
+#pragma unroll
+    for (int i = 1; i < L; i++) a.w[i] = 0;
+    return a;
+  }
+  return in_load<L>(in + lay.in_at(rp, k), lay.in_one_limb);
+}
+
 // Block (tile, chunk, sequence): out[c*bd + r] = x[o] + sum_{k>=1} f(r*k mod bd)
 // x[o + k 8^d] (radix8.rs:117-161), r in this block's tile of base rows.
-template <int L>
+// kChecked: every product and partial sum is checked in the reference's
+// fold order (k = 1, 2, ...; the k = 0 term x[o] * f(0) = x[o] is
+// from_ref and cannot overflow) and overflow bits are or-ed into *flag.
+template <int L, bool kChecked>
 __global__ void __launch_bounds__(kWideTile)
 wide_base_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
                  const int32_t* __restrict__ ftab, WideLayout lay, uint32_t bd, uint32_t bl,
-                 uint32_t depth, uint32_t nchunks, uint32_t ntiles) {
+                 uint32_t depth, uint32_t nchunks, uint32_t ntiles, uint32_t* __restrict__ flag) {
   __shared__ uint64_t xs[kWideTile * L];  // the chunk's inputs, kWideTile at a time
   const uint32_t tile = blockIdx.x % ntiles;
   const uint32_t c = (blockIdx.x / ntiles) % nchunks;
@@ -162,19 +244,25 @@ wide_base_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
   Big<L> acc;
 #pragma unroll
   for (int l = 0; l < L; l++) acc.w[l] = 0;
+  uint32_t ovf = 0;
   for (uint32_t k0 = 0; k0 < bl; k0 += kWideTile) {
     const uint32_t nk = min(kWideTile, bl - k0);
     __syncthreads();
     for (uint32_t k = threadIdx.x; k < nk; k += blockDim.x)
-      big_store(xs + k * L,
-                in_load<L>(in + lay.in_at(rp, (uint64_t)o + (uint64_t)(k0 + k) * nchunks),
-                           lay.in_one_limb));
+      big_store(xs + k * L, lay_load<L>(in, lay, rp, (uint64_t)o + (uint64_t)(k0 + k) * nchunks));
     __syncthreads();
-    if (r < bd)  // f(0) = 1 is from_ref for k = 0
-      for (uint32_t k = 0; k < nk; k++)
-        big_mac(acc, big_load<L>(xs + k * L), ftab[(r * (k0 + k)) & (bd - 1)]);
+    if (r < bd) {  // f(0) = 1 is from_ref for k = 0
+      for (uint32_t k = 0; k < nk; k++) {
+        const int32_t m = ftab[(r * (k0 + k)) & (bd - 1)];
+        if (kChecked)
+          ovf |= big_mac_checked(acc, big_load<L>(xs + k * L), m);
+        else
+          big_mac(acc, big_load<L>(xs + k * L), m);
+      }
+    }
   }
   if (r < bd) big_store(out + lay.out_at(rp, (uint64_t)c * bd + r), acc);
+  if (kChecked && ovf) atomicOr(flag, ovf);
 }
 
 // ---- Base layer on the matrix cores (depth 1 or 2, bl <= 256) ----------
@@ -380,10 +468,10 @@ int launch_wide_mfma(const zp_code* cd, const uint64_t* in, uint64_t seqs, const
 // j*sub + i of a block of 8*sub are replaced by their radix-8 butterfly.
 // The plane of a sequence is the fastest thread index: the planes of one
 // element are adjacent in memory, so a wave's loads and stores coalesce.
-template <int L>
+template <int L, bool kChecked>
 __global__ void __launch_bounds__(256)
 wide_stage_kernel(uint64_t* __restrict__ data, const int32_t* __restrict__ tw, WideLayout lay,
-                  uint64_t N, uint64_t sub, uint64_t total) {
+                  uint64_t N, uint64_t sub, uint64_t total, uint32_t* __restrict__ flag) {
   const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= total) return;
   const uint64_t per_row = N / 8;
@@ -397,18 +485,27 @@ wide_stage_kernel(uint64_t* __restrict__ data, const int32_t* __restrict__ tw, W
   // All 56 coefficients of this position are loaded at once (a stage has
   // few threads, so one load round trip per output would be exposed); the
   // coefficient of input j in output r is tw[(j-1)*8 + (r*j mod 8)]
-  // (butterfly.rs:8-17).
+  // (butterfly.rs:8-17).  kChecked: the fold over j = 1..7 is checked in
+  // that order (butterfly.rs:34-42).
   const int32_t* twi = tw + i * 56;
   int32_t T[56];
 #pragma unroll
   for (int e = 0; e < 56; e++) T[e] = twi[e];
+  uint32_t ovf = 0;
 #pragma unroll
   for (int r = 0; r < 8; r++) {
     Big<L> a = xs[0];
 #pragma unroll
-    for (int jm1 = 0; jm1 < 7; jm1++) big_mac(a, xs[jm1 + 1], T[jm1 * 8 + ((r * (jm1 + 1)) & 7)]);
+    for (int jm1 = 0; jm1 < 7; jm1++) {
+      const int32_t m = T[jm1 * 8 + ((r * (jm1 + 1)) & 7)];
+      if (kChecked)
+        ovf |= big_mac_checked(a, xs[jm1 + 1], m);
+      else
+        big_mac(a, xs[jm1 + 1], m);
+    }
     big_store(data + lay.out_at(rp, first + (uint64_t)r * sub), a);
   }
+  if (kChecked && ovf) atomicOr(flag, ovf);
 }
 
 constexpr uint32_t kScanThreads = 1024;
@@ -477,27 +574,147 @@ wide_raa_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
   }
 }
 
+// The checked encoding (raa.rs:205-218 accumulate with add!, and
+// raa_sign_flip.rs:146-155 with neg!).  Sums run in L + 1 limbs, exact for
+// any N < 2^64: every prefix of the accumulate must fit Int<L> (the first
+// prefix that does not is where the sequential checked fold panics), and
+// only Int::MIN fails to negate.  Values are stored in L limbs; they equal
+// the unchecked ones wherever nothing overflowed.
+template <int L>
+__device__ __forceinline__ Big<L + 1> big_widen(const Big<L>& a) {
+  Big<L + 1> w;
+#pragma unroll
+  for (int i = 0; i < L; i++) w.w[i] = a.w[i];
+  w.w[L] = big_is_neg(a) ? ~0ull : 0ull;
+  return w;
+}
+
+template <int L>
+__device__ __forceinline__ Big<L> big_narrow(const Big<L + 1>& a) {
+  Big<L> n;
+#pragma unroll
+  for (int i = 0; i < L; i++) n.w[i] = a.w[i];
+  return n;
+}
+
+template <int L>
+__device__ __forceinline__ bool big_fits(const Big<L + 1>& a) {
+  return a.w[L] == ((int64_t)a.w[L - 1] < 0 ? ~0ull : 0ull);
+}
+
+template <int L, bool kFlip>
+__global__ void __launch_bounds__(kScanThreads)
+wide_raa_checked_kernel(const uint64_t* __restrict__ in, uint64_t* __restrict__ out,
+                        uint64_t* __restrict__ amid, const uint32_t* __restrict__ perm1,
+                        const uint32_t* __restrict__ perm2, WideLayout lay, uint32_t row_len,
+                        uint32_t N, uint32_t* __restrict__ flag) {
+  constexpr int LS = L + 1;
+  __shared__ uint64_t part[kScanThreads * LS];
+  const uint64_t rp = blockIdx.x;
+  uint64_t* a = amid + rp * N * L;
+  const uint32_t per = (N + kScanThreads - 1) / kScanThreads;
+  const uint32_t i0 = min(N, threadIdx.x * per), i1 = min(N, i0 + per);
+  uint32_t ovf = 0;
+  auto y1 = [&](uint32_t i) {
+    const uint32_t s = perm1[i];
+    Big<L> v = lay_load<L>(in, lay, rp, s & (row_len - 1));
+    if (kFlip && (s & 1u)) ovf |= big_neg_checked(v);
+    return big_widen(v);
+  };
+  Big<LS> sum, total;
+#pragma unroll
+  for (int l = 0; l < LS; l++) sum.w[l] = 0;
+  for (uint32_t i = i0; i < i1; i++) big_add(sum, y1(i));
+  Big<LS> run = block_exscan_big(sum, part, &total);
+  for (uint32_t i = i0; i < i1; i++) {
+    big_add(run, y1(i));
+    if (!big_fits<L>(run)) ovf |= kOvfAdd;
+    Big<L> v = big_narrow<L>(run);
+    if (kFlip && (i & 1u)) ovf |= big_neg_checked(v);
+    big_store(a + (uint64_t)i * L, v);
+  }
+  __threadfence_block();
+  __syncthreads();
+#pragma unroll
+  for (int l = 0; l < LS; l++) sum.w[l] = 0;
+  for (uint32_t i = i0; i < i1; i++)
+    big_add(sum, big_widen(big_load<L>(a + (uint64_t)perm2[i] * L)));
+  run = block_exscan_big(sum, part, &total);
+  for (uint32_t i = i0; i < i1; i++) {
+    big_add(run, big_widen(big_load<L>(a + (uint64_t)perm2[i] * L)));
+    if (!big_fits<L>(run)) ovf |= kOvfAdd;
+    big_store(out + lay.out_at(rp, i), big_narrow<L>(run));
+  }
+  if (ovf) atomicOr(flag, ovf);
+}
+
+// Sign-normalised magnitude of every input element: for each limb position
+// l, the or over elements x of (x < 0 ? ~x : x) limb l, into mag[l].  An
+// element whose normalised value has its highest set bit at position b has
+// |x| <= 2^(b+1).  Elements are kLimbs words apart, contiguous.
+template <int kLimbs>
+__global__ void __launch_bounds__(256)
+magnitude_kernel(const uint64_t* __restrict__ in, uint64_t count,
+                 unsigned long long* __restrict__ mag) {
+  unsigned long long acc[kLimbs];
+#pragma unroll
+  for (int l = 0; l < kLimbs; l++) acc[l] = 0;
+  for (uint64_t e = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+       e += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t* p = in + e * kLimbs;
+    const uint64_t sx = (int64_t)p[kLimbs - 1] < 0 ? ~0ull : 0ull;
+#pragma unroll
+    for (int l = 0; l < kLimbs; l++) acc[l] |= p[l] ^ sx;
+  }
+#pragma unroll
+  for (int l = 0; l < kLimbs; l++) {
+    unsigned long long v = acc[l];
+    for (int off = 32; off > 0; off >>= 1) v |= __shfl_xor(v, off);
+    if ((threadIdx.x & 63) == 0 && v) atomicOr(mag + l, v);
+  }
+}
+
+// Words of the check buffer: the magnitude limbs, then the overflow flag.
+constexpr uint32_t kCheckWords = 9;
+
+__global__ void zero_words_kernel(unsigned long long* w, uint32_t n) {
+  if (threadIdx.x < n) w[threadIdx.x] = 0;
+}
+
+// flag != nullptr: the checked kernels (per-element base layer, checked
+// stages, checked accumulates), or-ing overflow bits into *flag.
 template <int L>
 int launch_wide(const zp_code* cd, const uint64_t* in, uint64_t seqs, const WideLayout& lay,
-                uint64_t* out, hipStream_t s) {
+                uint64_t* out, hipStream_t s, uint32_t* flag) {
   const uint32_t row_len = (uint32_t)cd->row_len, N = (uint32_t)cd->N;
   if (cd->kind == ZP_CODE_IPRS) {
     const uint32_t nchunks = 1u << (3 * cd->depth);
     const uint32_t ntiles = (cd->bd + kWideTile - 1) / kWideTile;
     const uint64_t blocks = seqs * nchunks * ntiles;
-    if (wide_mfma_eligible(cd)) {
+    if (!flag && wide_mfma_eligible(cd) && !lay.in_bits) {
       ZP_TRY(launch_wide_mfma<L>(cd, in, seqs, lay, out, s));
     } else {
       if (blocks > 0x7fffffffull) return fail(ZP_ERR_INVALID_PARAM, "too many rows");
-      hipLaunchKernelGGL((wide_base_kernel<L>), dim3((uint32_t)blocks), dim3(kWideTile), 0, s,
-                         in, out, cd->d_ftab, lay, cd->bd, cd->bl, cd->depth, nchunks, ntiles);
+      if (flag)
+        hipLaunchKernelGGL((wide_base_kernel<L, true>), dim3((uint32_t)blocks), dim3(kWideTile),
+                           0, s, in, out, cd->d_ftab, lay, cd->bd, cd->bl, cd->depth, nchunks,
+                           ntiles, flag);
+      else
+        hipLaunchKernelGGL((wide_base_kernel<L, false>), dim3((uint32_t)blocks), dim3(kWideTile),
+                           0, s, in, out, cd->d_ftab, lay, cd->bd, cd->bl, cd->depth, nchunks,
+                           ntiles, flag);
       ZP_HIP(hipGetLastError());
     }
     for (uint32_t k = 0; k < cd->depth; k++) {
       const uint64_t sub = (uint64_t)cd->bd << (3 * k);
       const uint64_t total = seqs * (N / 8);
-      hipLaunchKernelGGL((wide_stage_kernel<L>), dim3((uint32_t)((total + 255) / 256)), dim3(256),
-                         0, s, out, cd->d_tw[k], lay, (uint64_t)N, sub, total);
+      const dim3 grid((uint32_t)((total + 255) / 256));
+      if (flag)
+        hipLaunchKernelGGL((wide_stage_kernel<L, true>), grid, dim3(256), 0, s, out, cd->d_tw[k],
+                           lay, (uint64_t)N, sub, total, flag);
+      else
+        hipLaunchKernelGGL((wide_stage_kernel<L, false>), grid, dim3(256), 0, s, out,
+                           cd->d_tw[k], lay, (uint64_t)N, sub, total, flag);
       ZP_HIP(hipGetLastError());
     }
     return ZP_OK;
@@ -505,30 +722,116 @@ int launch_wide(const zp_code* cd, const uint64_t* in, uint64_t seqs, const Wide
   if (seqs > 0x7fffffffull) return fail(ZP_ERR_INVALID_PARAM, "too many rows");
   uint64_t* amid = nullptr;
   ZP_TRY(dev_alloc((void**)&amid, seqs * N * L * sizeof(uint64_t), s));
-  if (cd->kind == ZP_CODE_RAA_SIGNFLIP)
-    hipLaunchKernelGGL((wide_raa_kernel<L, true>), dim3((uint32_t)seqs), dim3(kScanThreads), 0, s,
-                       in, out, amid, cd->d_perm1, cd->d_perm2, lay, row_len, N);
+  const bool flip = cd->kind == ZP_CODE_RAA_SIGNFLIP;
+  const dim3 grid((uint32_t)seqs), block(kScanThreads);
+  if (flag && flip)
+    hipLaunchKernelGGL((wide_raa_checked_kernel<L, true>), grid, block, 0, s, in, out, amid,
+                       cd->d_perm1, cd->d_perm2, lay, row_len, N, flag);
+  else if (flag)
+    hipLaunchKernelGGL((wide_raa_checked_kernel<L, false>), grid, block, 0, s, in, out, amid,
+                       cd->d_perm1, cd->d_perm2, lay, row_len, N, flag);
+  else if (flip)
+    hipLaunchKernelGGL((wide_raa_kernel<L, true>), grid, block, 0, s, in, out, amid, cd->d_perm1,
+                       cd->d_perm2, lay, row_len, N);
   else
-    hipLaunchKernelGGL((wide_raa_kernel<L, false>), dim3((uint32_t)seqs), dim3(kScanThreads), 0,
-                       s, in, out, amid, cd->d_perm1, cd->d_perm2, lay, row_len, N);
+    hipLaunchKernelGGL((wide_raa_kernel<L, false>), grid, block, 0, s, in, out, amid,
+                       cd->d_perm1, cd->d_perm2, lay, row_len, N);
   const int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "launch failed");
   dev_free(amid, s);
   return rc;
 }
 
 int dispatch_wide(const zp_code* cd, uint32_t L, const uint64_t* in, uint64_t seqs,
-                  const WideLayout& lay, uint64_t* out, hipStream_t s) {
+                  const WideLayout& lay, uint64_t* out, hipStream_t s, uint32_t* flag = nullptr) {
   switch (L) {
-    case 1: return launch_wide<1>(cd, in, seqs, lay, out, s);
-    case 2: return launch_wide<2>(cd, in, seqs, lay, out, s);
-    case 3: return launch_wide<3>(cd, in, seqs, lay, out, s);
-    case 4: return launch_wide<4>(cd, in, seqs, lay, out, s);
-    case 5: return launch_wide<5>(cd, in, seqs, lay, out, s);
-    case 6: return launch_wide<6>(cd, in, seqs, lay, out, s);
-    case 7: return launch_wide<7>(cd, in, seqs, lay, out, s);
-    case 8: return launch_wide<8>(cd, in, seqs, lay, out, s);
+    case 1: return launch_wide<1>(cd, in, seqs, lay, out, s, flag);
+    case 2: return launch_wide<2>(cd, in, seqs, lay, out, s, flag);
+    case 3: return launch_wide<3>(cd, in, seqs, lay, out, s, flag);
+    case 4: return launch_wide<4>(cd, in, seqs, lay, out, s, flag);
+    case 5: return launch_wide<5>(cd, in, seqs, lay, out, s, flag);
+    case 6: return launch_wide<6>(cd, in, seqs, lay, out, s, flag);
+    case 7: return launch_wide<7>(cd, in, seqs, lay, out, s, flag);
+    case 8: return launch_wide<8>(cd, in, seqs, lay, out, s, flag);
   }
   return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 1..8");
+}
+
+int launch_magnitude(const uint64_t* in, uint64_t count, uint32_t limbs, unsigned long long* mag,
+                     hipStream_t s) {
+  if (count == 0) return ZP_OK;
+  const uint32_t blocks = (uint32_t)std::min<uint64_t>(2048, (count + 255) / 256);
+  switch (limbs) {
+#define ZP_MAG(n) \
+  case n: hipLaunchKernelGGL((magnitude_kernel<n>), dim3(blocks), dim3(256), 0, s, in, count, mag); break;
+    ZP_MAG(1) ZP_MAG(2) ZP_MAG(3) ZP_MAG(4) ZP_MAG(5) ZP_MAG(6) ZP_MAG(7) ZP_MAG(8)
+#undef ZP_MAG
+    default: return fail(ZP_ERR_INVALID_PARAM, "CombR limbs must be within 1..8");
+  }
+  ZP_HIP(hipGetLastError());
+  return ZP_OK;
+}
+
+// The checked encoding of `seqs` sequences (eval_type < 0: encode_wide
+// rows of Int<L>).  Tier 1: if inputs of the type's full width cannot
+// overflow the Cw type through this code (code_growth_log2), nothing is
+// checked.  Tier 2 (integer inputs): a scan of the inputs' magnitudes
+// decides the same for these inputs.  Tier 3: the checked kernels encode
+// into `out` and the host reads their overflow bits.  *done tells the
+// caller whether `out` holds the encoding (tiers 1 and 2 leave it to the
+// ordinary encoder, whose result then equals the checked one).
+int run_checked(const zp_code* cd, uint32_t L, const uint64_t* in, uint64_t seqs,
+                const WideLayout& lay, uint64_t* out, hipStream_t s, double cw_bits,
+                double in_bits, uint64_t scan_count, uint32_t scan_limbs, bool* done) {
+  *done = false;
+  const double growth = code_growth_log2(cd);
+  if (in_bits + growth + 1e-9 < cw_bits - 1) return ZP_OK;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  ZP_HIP(hipStreamIsCapturing(s, &cs));
+  if (cs != hipStreamCaptureStatusNone)
+    return fail(ZP_ERR_INVALID_PARAM,
+                "a checked code cannot encode these inputs under stream capture: its overflow "
+                "check reads a result on the host");
+  unsigned long long* buf = nullptr;
+  ZP_TRY(dev_alloc((void**)&buf, kCheckWords * 8, s));
+  unsigned long long host[kCheckWords] = {};
+  int rc = ZP_OK;
+  hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, buf, kCheckWords);
+  if (hipGetLastError() != hipSuccess) rc = fail(ZP_ERR_DEVICE, "launch failed");
+  bool small = false;
+  if (rc == ZP_OK && scan_limbs) {
+    rc = launch_magnitude(in, scan_count, scan_limbs, buf, s);
+    if (rc == ZP_OK && (hipMemcpyAsync(host, buf, 8 * 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess))
+      rc = fail(ZP_ERR_DEVICE, "magnitude read failed");
+    if (rc == ZP_OK) {
+      double bits = 0;
+      for (int l = (int)scan_limbs - 1; l >= 0; l--)
+        if (host[l]) {
+          bits = 64.0 * l + (64 - __builtin_clzll(host[l]));
+          break;
+        }
+      small = bits + growth + 1e-9 < cw_bits - 1;
+    }
+  }
+  if (rc == ZP_OK && !small) {
+    uint32_t* flag = (uint32_t*)(buf + 8);
+    rc = dispatch_wide(cd, L, in, seqs, lay, out, s, flag);
+    if (rc == ZP_OK && (hipMemcpyAsync(&host[8], buf + 8, 8, hipMemcpyDeviceToHost, s) != hipSuccess ||
+                        hipStreamSynchronize(s) != hipSuccess))
+      rc = fail(ZP_ERR_DEVICE, "overflow flag read failed");
+    if (rc == ZP_OK) {
+      *done = true;
+      const uint32_t f = (uint32_t)host[8];
+      if (f)
+        rc = fail(ZP_ERR_OVERFLOW,
+                  std::string(f & kOvfMul ? "Multiplication by twiddle should not overflow"
+                              : f & kOvfAdd ? (cd->kind == ZP_CODE_IPRS ? "Addition overflow"
+                                                                        : "Accumulation overflow")
+                                            : "Negation overflow"));
+    }
+  }
+  dev_free(buf, s);
+  return rc;
 }
 
 }  // namespace
@@ -537,7 +840,52 @@ int launch_encode_wide(const zp_code* cd, uint32_t L, const uint64_t* in, uint64
                        uint64_t* out, hipStream_t s) {
   if (rows == 0) return ZP_OK;
   const WideLayout lay{1, false, cd->row_len * L, 0, L, cd->N * L, 0, L};
+  if (cd->checked) {
+    bool done = false;
+    ZP_TRY(run_checked(cd, L, in, rows, lay, out, s, 64.0 * L, 64.0 * L - 1, rows * cd->row_len,
+                       L, &done));
+    if (done) return ZP_OK;
+  }
   return dispatch_wide(cd, L, in, rows, lay, out, s);
+}
+
+int encode_checked(const zp_code* cd, int eval_type, uint32_t D, const void* in, uint64_t rows,
+                   void* out, hipStream_t s, bool* done) {
+  *done = false;
+  if (rows == 0) return ZP_OK;
+  const bool iprs = cd->kind == ZP_CODE_IPRS;
+  const double growth = code_growth_log2(cd);
+  const uint64_t* w = (const uint64_t*)in;
+  uint64_t* o = (uint64_t*)out;
+  switch (eval_type) {
+    case ZP_EVAL_I32:  // |x| <= 2^31 into i128: no code here can overflow it
+      if (31 + growth + 1e-9 < 127) return ZP_OK;
+      return fail(ZP_ERR_INVALID_PARAM, "no checked encoder for i32 rows of this code");
+    case ZP_EVAL_I64: {  // Cw = i128 under either code
+      const WideLayout lay{1, true, cd->row_len, 1, 1, cd->N * 2, 2, 2};
+      return run_checked(cd, 2, w, rows, lay, o, s, 128, 63, rows * cd->row_len, 1, done);
+    }
+    case ZP_EVAL_DENSE_I64: {  // Cw = [i64; D] (IPRS) or [Int<4>; D] (RAA)
+      const uint32_t L = iprs ? 1 : 4;
+      const WideLayout lay{D, true, cd->row_len * D, 1, D, cd->N * D * L, L, (uint64_t)D * L};
+      return run_checked(cd, L, w, rows * D, lay, o, s, 64.0 * L, 63, rows * cd->row_len * D, 1,
+                         done);
+    }
+    case ZP_EVAL_BINPOLY: {
+      // Coefficients are bits (|x| <= 1): Cw = [i64; D] under IPRS, whose
+      // planes the checked kernels encode one by one; Cw = [i32; D] under
+      // RAA, which the code's width check at creation keeps from
+      // overflowing (raa.rs:52-69).
+      if (!iprs) {
+        if (growth + 1e-9 < 31) return ZP_OK;
+        return fail(ZP_ERR_INVALID_PARAM, "no checked encoder for binary rows of this code");
+      }
+      WideLayout lay{D, false, cd->row_len, 0, 1, cd->N * D, 1, D};
+      lay.in_bits = true;
+      return run_checked(cd, 1, w, rows * D, lay, o, s, 64, 0, 0, 0, done);
+    }
+  }
+  return fail(ZP_ERR_INVALID_PARAM, "unknown eval type");
 }
 
 // Rows of i64 evals (Eval = i64, Cw = i128: L = 2) or of

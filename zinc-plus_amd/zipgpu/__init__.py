@@ -32,7 +32,7 @@ import torch
 from . import _lib as L
 
 __all__ = [
-    "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory",
+    "ZipError", "InvalidPcsParam", "ShapeError", "DeviceError", "OutOfMemory", "Overflow",
     "ZipTypes", "BinPolyTypes", "IntTypes", "I64Types", "DensePolyTypes", "IprsCode", "RaaCode", "RaaSignFlippingCode",
     "ZipPlus", "ZipPlusParams", "ZipPlusHint", "ZipPlusCommitment", "MerkleProof",
     "merkle_root", "lib", "set_option", "BatchCommit", "CommitServer", "PcsProverTranscript",
@@ -65,8 +65,15 @@ class OutOfMemory(ZipError):
     """Device allocation failed."""
 
 
+class Overflow(ZipError):
+    """A checked code's encoding left its Cw type: where the reference's
+    checked build panics ("Multiplication by twiddle should not overflow",
+    add!'s "Addition overflow", "Accumulation overflow", neg!)."""
+
+
 _ERRORS = {L.ZP_ERR_INVALID_PARAM: InvalidPcsParam, L.ZP_ERR_SHAPE: ShapeError,
-           L.ZP_ERR_DEVICE: DeviceError, L.ZP_ERR_OOM: OutOfMemory}
+           L.ZP_ERR_DEVICE: DeviceError, L.ZP_ERR_OOM: OutOfMemory,
+           L.ZP_ERR_OVERFLOW: Overflow}
 
 
 def lib():
@@ -176,7 +183,7 @@ class LinearCode:
 
     KIND = None
 
-    def __init__(self, kind, rep, row_len, depth):
+    def __init__(self, kind, rep, row_len, depth, checked=False):
         self._h = None
         lib_ = L.load()
         h = ctypes.c_void_p()
@@ -184,6 +191,14 @@ class LinearCode:
         self._h = h
         self._lib = lib_
         self.REPETITION_FACTOR = rep
+        if checked:
+            _check(lib_.zp_code_set_checked(h, 1))
+
+    @property
+    def checked(self):
+        """The CHECK const (IprsCode<.., CHECK>, RaaConfig::CHECK_FOR_OVERFLOWS):
+        a checked code's encodings raise Overflow where the reference panics."""
+        return bool(self._lib.zp_code_is_checked(self._h))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -276,13 +291,13 @@ class IprsCode(LinearCode):
 
     KIND = L.ZP_CODE_IPRS
 
-    def __init__(self, row_len, depth, rep=4):
-        super().__init__(L.ZP_CODE_IPRS, rep, row_len, depth)
+    def __init__(self, row_len, depth, rep=4, checked=False):
+        super().__init__(L.ZP_CODE_IPRS, rep, row_len, depth, checked)
         self.depth = int(self._lib.zp_code_depth(self._h))
 
     @classmethod
-    def new_with_optimal_depth(cls, row_len, rep=4):
-        return cls(row_len, -1, rep)
+    def new_with_optimal_depth(cls, row_len, rep=4, checked=False):
+        return cls(row_len, -1, rep, checked)
 
     def params_string(self):
         return f"row_len={self.row_len()}, rate=1/{self.REPETITION_FACTOR}, depth={self.depth}"
@@ -293,8 +308,8 @@ class RaaCode(LinearCode):
 
     KIND = L.ZP_CODE_RAA
 
-    def __init__(self, row_len, rep=4):
-        super().__init__(self.KIND, rep, row_len, 0)
+    def __init__(self, row_len, rep=4, checked=False):
+        super().__init__(self.KIND, rep, row_len, 0, checked)
 
     def params_string(self):
         return f"row_len={self.row_len()}, rate=1/{self.REPETITION_FACTOR}"

@@ -70,6 +70,8 @@ SIGNATURES = {
     "zp_set_option": (_int, [ctypes.c_char_p, ctypes.c_int64]),
     "zp_code_create": (_int, [_int, _u32, _u64, _i32, ctypes.POINTER(_c_void_p)]),
     "zp_code_destroy": (None, [_c_void_p]),
+    "zp_code_set_checked": (_int, [_c_void_p, _int]),
+    "zp_code_is_checked": (_int, [_c_void_p]),
     "zp_code_row_len": (_u64, [_c_void_p]),
     "zp_code_codeword_len": (_u64, [_c_void_p]),
     "zp_code_depth": (_u32, [_c_void_p]),
