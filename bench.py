@@ -11,6 +11,9 @@ Also reported on the same line:
   roofline      the dominant kernel (the matrix-core IPRS encoder) timed with
                 HIP events on the stream it runs on; traffic from the
                 committed rocprofv3 counter summary when present
+  commit_roofline  the floor of the whole commit step: its HBM bytes and its
+                VALU operations (BLAKE3 at its per-compression minimum, the
+                radix-8 stages' multiply-adds) at the published peaks
   cpu_baseline  the reference's CPU commit of the same workload, restated on
                 its parallel axes with its AVX-512 paths (all allotted host
                 threads, and one thread)
@@ -44,6 +47,37 @@ HBM_PEAK_GBS = 8000.0
 ENCODE_BYTES_PER_EVAL = 8 + REP * D * 8
 ENCODE_KERNEL = "bitplane_kernel + iprs_mfma_kernel (one encode_rows)"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "iprs_mfma_kernel_commit_2p20_d32_pmc.txt")
+# VALU lane-operations per second at the published clock: 256 CUs x 64 lanes
+# x 2.4 GHz of int32 or FP64 issue (FP64 vector 78.6 TFLOP/s = FMA at this
+# rate).  BLAKE3's compression takes at least 688 such operations on gfx950
+# (7 rounds x 8 G functions x 12: v_add3_u32, add, xor + v_alignbit_b32
+# rotations, and the 16 output xors).
+VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9
+BLAKE3_OPS_PER_COMPRESSION = 688
+
+
+def commit_roofline(ms_per_step, num_rows, N):
+    """The floor of one whole commit step: HBM bytes (evaluations read,
+    codewords written by the encoder and read back by the leaf hashing,
+    chunk chaining values, Merkle layers) and VALU lane-operations (every
+    BLAKE3 compression at its 688-op minimum, and the radix-8 stages at 7
+    multiply-adds per output per stage); the step cannot beat the larger."""
+    elem = D * 8
+    cw = num_rows * N * elem
+    leaf_bytes = num_rows * elem
+    chunks = (leaf_bytes + 1023) // 1024
+    hbm = N_EVALS * 8 + 2 * cw + 2 * N * chunks * 32 + (2 * N - 1) * 32
+    blocks = cw // 64                      # 64-byte blocks of every leaf
+    parents = N * (chunks - 1) + (N - 1)   # chunk-tree and Merkle parents
+    depth = 2
+    stage_fma = 7 * depth * num_rows * N * D
+    valu = BLAKE3_OPS_PER_COMPRESSION * (blocks + parents) + stage_fma
+    hbm_ms = hbm / (HBM_PEAK_GBS * 1e9) * 1e3
+    valu_ms = valu / VALU_LANE_OPS_PEAK * 1e3
+    floor = max(hbm_ms, valu_ms)
+    return {"bound": "valu" if valu_ms >= hbm_ms else "hbm", "hbm_bytes": hbm,
+            "hbm_floor_ms": hbm_ms, "valu_lane_ops": valu, "valu_floor_ms": valu_ms,
+            "floor_ms": floor, "frac": floor / ms_per_step}
 
 
 def log(*a):
@@ -589,6 +623,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_per_launch(),
                      "launch_ms": enc_ms, "bytes_per_launch": enc_bytes,
                      "valu_encoder_launch_ms": valu_ms},
+        "commit_roofline": commit_roofline(ms, pp.num_rows, code.codeword_len()),
         "host_input_commit": pcie,
     }
     if not args.no_legs:
