@@ -49,18 +49,18 @@ ENCODE_KERNEL = "bitplane_kernel + iprs_mfma_kernel (one encode_rows)"
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "iprs_mfma_kernel_commit_2p20_d32_pmc.txt")
 # VALU lane-operations per second at the published clock: 256 CUs x 64 lanes
 # x 2.4 GHz of int32 or FP64 issue (FP64 vector 78.6 TFLOP/s = FMA at this
-# rate).  BLAKE3's compression takes at least 688 such operations on gfx950
-# (7 rounds x 8 G functions x 12: v_add3_u32, add, xor + v_alignbit_b32
-# rotations, and the 16 output xors).
+# rate).  A BLAKE3 compression that yields a chaining value takes at least
+# 680 such operations on gfx950 (7 rounds x 8 G functions x 12: v_add3_u32,
+# add, xor + v_alignbit_b32 rotations; then 8 output xors).
 VALU_LANE_OPS_PEAK = 256 * 64 * 2.4e9
-BLAKE3_OPS_PER_COMPRESSION = 688
+BLAKE3_OPS_PER_COMPRESSION = 680
 
 
 def commit_roofline(ms_per_step, num_rows, N):
     """The floor of one whole commit step: HBM bytes (evaluations read,
     codewords written by the encoder and read back by the leaf hashing,
     chunk chaining values, Merkle layers) and VALU lane-operations (every
-    BLAKE3 compression at its 688-op minimum, and the radix-8 stages at 7
+    BLAKE3 compression at its 680-op minimum, and the radix-8 stages at 7
     multiply-adds per output per stage); the step cannot beat the larger."""
     elem = D * 8
     cw = num_rows * N * elem
