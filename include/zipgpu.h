@@ -253,7 +253,10 @@ typedef struct {
 enum { ZP_BATCH_GRAPH = 1, ZP_BATCH_AUTOTUNE = 2 };
 
 /* flags: ZP_BATCH_GRAPH, ZP_BATCH_AUTOTUNE (run zp_autotune once per
- * distinct code at creation).  n_streams: branch streams (0 = 4). */
+ * distinct code at creation).  n_streams: branch streams (0 = 4).  A plan
+ * holding a checked code whose overflow check reads a verdict on the host
+ * (see zp_code_set_checked; mark codes before creating the plan) replays
+ * its commits eagerly from the arena instead of as a graph. */
 int zp_batch_create(const zp_commit_desc* items, uint32_t n_items, uint32_t flags,
                     uint32_t n_streams, zp_batch** out);
 /* d_evals[i]: item i's evals (batch x num_rows x row_len, poly-major).

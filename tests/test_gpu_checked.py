@@ -230,3 +230,36 @@ def test_checked_commit_and_noalloc(zg, dev, oracle):
         assert rc == 0
         plain = zg.IprsCode(4096, depth)
         assert torch.equal(cw, plain.encode_rows(ev, 1, zg.BinPolyTypes(D)).reshape(N, D))
+
+
+def test_checked_codes_in_batch_plans_and_autotune(zg, dev, oracle):
+    # A graph plan holding a checked depth-3 binary code (its check reads a
+    # verdict on the host) replays eagerly: the roots equal the oracle's over
+    # three commits, and rows that overflow raise Overflow.  The autotuner
+    # times a checked dense code's variants on synthetic rows that overflow,
+    # without failing, and leaves the code checked.
+    say("checked batch plan")
+    D, row_len = 32, 512
+    code3 = zg.IprsCode(row_len, 3, checked=True)
+    code2 = zg.IprsCode(1 << 12, 2, checked=True)
+    types = zg.BinPolyTypes(D)
+    items = [(zg.ZipPlus(types).setup(row_len * 2, code3), types),
+             (zg.ZipPlus(types).setup(1 << 14, code2), types)]
+    plan = zg.BatchCommit(items, graph=True)
+    xs = [synth.binary_polys(40, row_len * 2, D), synth.binary_polys(41, 1 << 14, D)]
+    want = [oracle.commit("iprs", xs[0], 2, row_len, depth=3, D=D)["root"],
+            oracle.commit("iprs", xs[1], 4, 1 << 12, depth=2, D=D)["root"]]
+    ev = [torch.from_numpy(x).to(dev) for x in xs]
+    for rnd in range(3):
+        got = plan.commit(ev)
+        assert [c.root for c in got] == want, rnd
+    big = zg.BatchCommit([(zg.ZipPlus(types).setup(4096 * 2, zg.IprsCode(4096, 3, checked=True)),
+                           types)], graph=True)
+    wide = to_dev(synth.splitmix64(5, 4096 * 2), dev)
+    with pytest.raises(zg.Overflow):
+        big.commit([wide])
+
+    say("checked autotune")
+    dense = zg.IprsCode(1024, 2, checked=True)
+    dense.autotune(zg.DensePolyTypes(4), 4)
+    assert dense.checked

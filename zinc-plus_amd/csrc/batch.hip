@@ -39,6 +39,9 @@ struct zp_batch {
   bool measured = false;
   hipGraphExec_t exec = nullptr;
   std::vector<const void*> inputs;  // the input addresses the graph was captured with
+  // A checked code whose overflow check reads a verdict on the host cannot
+  // be captured: such a plan replays its commits eagerly.
+  bool eager_only = false;
 };
 
 namespace zpi {
@@ -149,6 +152,15 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
                 zp_stream stream) {
   hipStream_t s = (hipStream_t)stream;
   if (!code) return fail(ZP_ERR_INVALID_PARAM, "null code");
+  // The variants are timed on synthetic rows with the ordinary kernels: a
+  // checked code's overflow check does not depend on the variant, and the
+  // synthetic values may well overflow.
+  struct Unchecked {
+    zp_code* c;
+    bool was;
+    ~Unchecked() { c->checked = was; }
+  } unchecked{code, code->checked};
+  code->checked = false;
   zp_hint* h = nullptr;
   ZP_TRY(hint_alloc(code, eval_type, D, num_rows, batch, s, &h));
   const uint64_t words = num_rows * batch * code->row_len * eval_bytes_of(eval_type, D) / 8;
@@ -248,6 +260,7 @@ int zp_batch_create(const zp_commit_desc* items, uint32_t n_items, uint32_t flag
     else if (d.num_rows == 0) rc = fail(ZP_ERR_SHAPE, "num_rows must be positive");
     if (rc == ZP_OK) rc = check_types(d.code, d.eval_type, d.degree_plus_one);
     if (rc != ZP_OK) break;
+    if (checked_may_sync(d.code, d.eval_type, d.degree_plus_one)) b->eager_only = true;
     h.device = b->device;
     h.owned = false;
     h.N = d.code->N;
@@ -364,7 +377,7 @@ int zp_batch_commit(zp_batch* b, const void* const* d_evals, uint8_t* h_roots, z
       b->arena.size = m.used;
       b->measured = true;
     }
-  } else if (b->flags & ZP_BATCH_GRAPH) {
+  } else if ((b->flags & ZP_BATCH_GRAPH) && !b->eager_only) {
     if (!b->exec || !std::equal(b->inputs.begin(), b->inputs.end(), d_evals))
       rc = capture(b, d_evals);
     if (rc == ZP_OK && hipGraphLaunch(b->exec, s) != hipSuccess)

@@ -184,6 +184,9 @@ int launch_encode_i64_planes(const zp_code* c, uint32_t D, uint32_t L, const int
 int encode_checked(const zp_code* c, int eval_type, uint32_t D, const void* in, uint64_t rows,
                    void* out, hipStream_t s, bool* done);
 inline double code_growth_log2(const zp_code* c) { return c->growth_log2; }
+// True when a checked code's encoding of these rows may need the checks
+// that read a verdict on the host (so it cannot run under stream capture).
+bool checked_may_sync(const zp_code* c, int eval_type, uint32_t D);
 // prove.hip: column values and Merkle paths of n_cols columns of a hint
 // (columns: n_cols x rows x elem bytes; paths: n_cols x (height-1) x 32).
 int launch_gather_openings(const zp_hint* h, const uint64_t* d_cols, uint32_t n_cols,

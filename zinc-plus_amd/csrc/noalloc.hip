@@ -70,7 +70,7 @@ extern "C" int zp_commit_noalloc(const zp_code* code, int eval_type, uint32_t D,
   ZP_TRY(check_types(code, eval_type, D));
   // A checked IPRS code whose binary rows can overflow i64 (depth 3) is
   // checked by kernels whose result the host reads back: not here.
-  if (code->checked && code->kind == ZP_CODE_IPRS && !(code_growth_log2(code) + 1e-9 < 63))
+  if (checked_may_sync(code, eval_type, D))
     return fail(ZP_ERR_INVALID_PARAM,
                 "zp_commit_noalloc takes no checked code whose encoding needs an overflow check");
   const uint64_t N = code->N;

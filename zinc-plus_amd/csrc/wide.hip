@@ -849,6 +849,19 @@ int launch_encode_wide(const zp_code* cd, uint32_t L, const uint64_t* in, uint64
   return dispatch_wide(cd, L, in, rows, lay, out, s);
 }
 
+bool checked_may_sync(const zp_code* cd, int eval_type, uint32_t D) {
+  (void)D;
+  if (!cd->checked) return false;
+  const bool iprs = cd->kind == ZP_CODE_IPRS;
+  const double g = code_growth_log2(cd) + 1e-9;
+  switch (eval_type) {
+    case ZP_EVAL_I64: return !(63 + g < 127);
+    case ZP_EVAL_DENSE_I64: return !(63 + g < (iprs ? 63 : 255));
+    case ZP_EVAL_BINPOLY: return iprs && !(g < 63);
+  }
+  return false;  // i32 rows: cleared by the bound, or refused, without a sync
+}
+
 int encode_checked(const zp_code* cd, int eval_type, uint32_t D, const void* in, uint64_t rows,
                    void* out, hipStream_t s, bool* done) {
   *done = false;
