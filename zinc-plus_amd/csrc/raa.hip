@@ -423,10 +423,15 @@ __device__ __forceinline__ void raa_row_body(const uint64_t* __restrict__ evals,
   auto rank = [&](uint32_t m) -> uint32_t {
     const uint32_t w = m >> 5;
     if constexpr (kFlip) {
-      const uint32_t all = rank_word(bp[w], m, (uint32_t)(int32_t)(int16_t)cpp[w]);
-      const uint32_t odd = rank_word(bp[w] & oddm[w], m, 0u);
-      const uint32_t v = all - 2u * odd;
-      return (m & 1u) ? 0u - v : v;
+      const uint32_t word = bp[w];
+      const uint32_t all = rank_word(word, m, (uint32_t)(int32_t)(int16_t)cpp[w]);
+      const uint32_t odd = rank_word(word & oddm[w], m, 0u);
+      // v = all - 2 odd in one 24-bit multiply-add (odd <= N <= 2^14), then
+      // negated at an odd position as (v ^ mask) - mask, mask = -(m & 1).
+      uint32_t v;
+      asm("v_mad_i32_i24 %0, %1, -2, %2" : "=v"(v) : "v"(odd), "v"(all));
+      const uint32_t mask = (uint32_t)((int32_t)(m << 31) >> 31);
+      return (v ^ mask) - mask;
     }
     return rank_word(bp[w], m, cpp[w]);
   };
