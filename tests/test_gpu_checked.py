@@ -263,3 +263,24 @@ def test_checked_codes_in_batch_plans_and_autotune(zg, dev, oracle):
     dense = zg.IprsCode(1024, 2, checked=True)
     dense.autotune(zg.DensePolyTypes(4), 4)
     assert dense.checked
+
+
+def test_checked_eager_plan_scratch_follows_the_inputs(zg, dev, oracle):
+    # A plan with a checked dense item: the first commit's rows need the
+    # checked kernels (|x| < 2^12 at depth 2 is above the bound but fits),
+    # the second's are cleared by the magnitude scan and take the
+    # matrix-core base layer, whose scratch the first commit never used.
+    # Both commits equal the plain commits' roots.
+    say("checked plan, inputs change tier")
+    rows, row_len, D = 4, 1024, 2
+    types = zg.DensePolyTypes(D)
+    pp = zg.ZipPlus(types).setup(rows * row_len, zg.IprsCode(row_len, 2, checked=True))
+    plain_pp = zg.ZipPlus(types).setup(rows * row_len, zg.IprsCode(row_len, 2))
+    plan = zg.BatchCommit([(pp, types)], graph=True)
+    for shift in (52, 57, 52, 57):
+        x = torch.from_numpy(synth.splitmix64(77 + shift, rows * row_len * D).view(np.int64)
+                             >> np.int64(shift)).to(dev)
+        got = plan.commit([x])[0].root
+        h, want = zg.ZipPlus(types).commit(plain_pp, [x])
+        h.free()
+        assert got == want.root, shift

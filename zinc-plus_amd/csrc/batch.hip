@@ -383,8 +383,13 @@ int zp_batch_commit(zp_batch* b, const void* const* d_evals, uint8_t* h_roots, z
     if (rc == ZP_OK && hipGraphLaunch(b->exec, s) != hipSuccess)
       rc = fail(ZP_ERR_DEVICE, "graph launch failed");
   } else {
-    b->arena.used = 0;
-    set_arena(&b->arena);
+    // An eager-only plan takes its scratch from the cache: a checked
+    // encoding's scratch depends on the inputs (the matrix-core fast path
+    // or the checked kernels), so the first commit's measure need not hold.
+    if (!b->eager_only) {
+      b->arena.used = 0;
+      set_arena(&b->arena);
+    }
     rc = enqueue_all(b, d_evals, s);
     set_arena(nullptr);
   }
