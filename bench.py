@@ -522,8 +522,10 @@ def copy_calibration():
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    # Commits of 0.7 ms: 40 warm-up steps (28 ms) bring the board's clocks
+    # to their steady state before 100 timed steps (70 ms).
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=40)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-legs", action="store_true", help="skip the other workloads")
     ap.add_argument("--legs", default="", help="comma-separated workload names to run (default all)")
