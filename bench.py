@@ -182,9 +182,9 @@ def leg_raa(torch, zipgpu, synth, dev):
     stream = torch.cuda.Stream()
     # Back-to-back launches from idle run fast for a few launches, slow down
     # for a few more and then settle (the board's power management), so the
-    # average spans 20 launches after 5.
+    # average spans 40 launches after 20.
     enc = event_ms(torch, lambda s: code.encode_rows(x, pp.num_rows, types, out=out, stream=s),
-                   stream, 20, warmup=5)
+                   stream, 40, warmup=20)
     del out
 
     def step():
@@ -211,7 +211,7 @@ def leg_raa_signflip(torch, zipgpu, synth, dev):
     out = torch.empty((pp.num_rows, code.codeword_len(), d), dtype=torch.int32, device=dev)
     stream = torch.cuda.Stream()
     enc = event_ms(torch, lambda s: code.encode_rows(x, pp.num_rows, types, out=out, stream=s),
-                   stream, 20, warmup=5)
+                   stream, 40, warmup=20)
     gbs = (8 + REP * d * 4) * n / (enc * 1e-3) / 1e9
     return {"workload": "RaaSignFlippingCode encode_rows 2^22 BinaryPoly<64>, row 2^11 x 2^11",
             "encode_ms": enc, "encode_gbs_algorithmic": gbs, "encode_hbm_frac": gbs / HBM_PEAK_GBS}
