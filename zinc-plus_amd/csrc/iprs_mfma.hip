@@ -67,15 +67,19 @@ constexpr uint64_t kMagicBits = 0x4338000000000000ull;
 constexpr double kMagicHi = kMagic - (double)(kMagicBits >> 32);
 static_assert((kMagicBits & 0xffffffffull) == 0, "bias must be a multiple of 2^32");
 
-// planes[((row*NC + c)*W + w)*D + p] holds the 16 bits of plane p for
+// planes[(((row*NC + c)*(D/16) + p/16)*W + w)*16 + p%16] holds the 16 bits of plane p for
 // k = 16w .. 16w+15 in "spread" order: bit i = 4v + b of the group sits at
 // position 8b + v, so (word >> v) & 0x01010101 is operand register v of the
 // MFMA B fragment (bytes b = 0..3 for k = 16w + 4v + b) in two instructions.
 // One thread per (row, c, w, 16 planes; D is a multiple of 16 on this
-// path), planes fastest: a lane writes its 16 words as four 16-byte stores
-// (the 2 lanes of a (c, w) cover 128 contiguous bytes for D = 32), and the
-// chunks whose octet reversals are adjacent follow, so one load instruction
-// reads adjacent evals.  With 4 planes per lane, eight lanes loaded the same
+// path): a lane writes its 16 words as four 16-byte stores, and the chunks
+// whose octet reversals are adjacent follow on neighbouring lanes, so one
+// load instruction reads adjacent evals.  Each group of 16 planes has its
+// own run of words, so the main kernel's loads read whole cache lines: with
+// the planes of a word interleaved, a workgroup read half of every line it
+// touched (the other plane group's half), and the extra line traffic
+// through the CU's L1 slowed the stores of the workgroup beside it
+// (0.268 -> 0.257 ms per 2^20-eval encode).  With 4 planes per lane, eight lanes loaded the same
 // 16 evals and the pre-pass was bound by load issue (12 -> ~8 µs at 2^20).
 constexpr uint32_t kPlanesPerLane = 16;
 template <int NC>
@@ -114,7 +118,7 @@ bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ plane
 #pragma unroll
     for (int q = 0; q < (int)PPL; q++) v[q] |= ((bits >> q) & 1u) << pos;
   }
-  uint4* dst = (uint4*)(planes + ((row * NC + c) * W + w) * D + PPL * pq);
+  uint4* dst = (uint4*)(planes + (((row * NC + c) * PQ + pq) * W + w) * PPL);
 #pragma unroll
   for (int q = 0; q < (int)PPL / 4; q++)
     dst[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
@@ -233,13 +237,13 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   // One row's plane words span NC*W*D*4 <= 256 KiB: 32-bit offsets from the
   // (uniform) row base.
   const char* prow = (const char*)(planes + row * NC * W * D);
-  const uint32_t poff = ((wave * W + g) * D + 16 * pg + pcol) * 4;
+  const uint32_t poff = (((wave * npg + pg) * W + g) * 16 + pcol) * 4;
   uint32_t pw[kTiles][S];
 #pragma unroll
   for (uint32_t i = 0; i < kTiles; i++)
 #pragma unroll
     for (int s = 0; s < S; s++)
-      pw[i][s] = *(const uint32_t*)(prow + (poff + (4 * i * W + 4 * s) * D * 4));
+      pw[i][s] = *(const uint32_t*)(prow + (poff + 4 * i * W * D * 4 + 4 * s * 64));
   v4i A[S][3];
 #pragma unroll
   for (int s = 0; s < S; s++)
