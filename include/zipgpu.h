@@ -269,6 +269,19 @@ int zp_batch_commit(zp_batch* plan, const void* const* d_evals, uint8_t* h_roots
 zp_hint* zp_batch_hint(zp_batch* plan, uint32_t item);
 int zp_batch_info(const zp_batch* plan, uint64_t* hint_bytes, uint64_t* scratch_bytes,
                   uint32_t* graph_ready);
+/* The plan's schedule (no reference counterpart): grouped = 1 when the
+ * leaf hashing of all items runs in grouped launches after the encoders
+ * (one persistent launch for every item's BLAKE3 chunks, one per
+ * chunk-tree shape, one per band of 8 Merkle levels), 0 when each item's
+ * hashing follows its encoder on its branch stream; streams = branch
+ * streams in use.  With ZP_BATCH_AUTOTUNE the first commit times every
+ * schedule (grouped or not, 2, 4 or 8 branches) on the plan's inputs and
+ * keeps the fastest; otherwise it is grouped on min(n_streams, items). */
+int zp_batch_schedule(const zp_batch* plan, uint32_t* grouped, uint32_t* streams);
+/* Sets the schedule (streams 1..the plan's branch streams); the next commit
+ * measures the scratch again and, with ZP_BATCH_GRAPH, captures anew.
+ * Results do not depend on the schedule. */
+int zp_batch_set_schedule(zp_batch* plan, uint32_t grouped, uint32_t streams);
 void zp_batch_destroy(zp_batch* plan);
 
 /* ---- Autotuner (no reference counterpart) --------------------------------

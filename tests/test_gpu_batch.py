@@ -229,3 +229,81 @@ def test_batch_i64_and_dense_items_match_plain_commits(zg, dev):
     for rep in range(3):
         assert [c.root for c in plan.commit(xs)] == want, rep
         eager_noise(zg, dev)
+
+
+# Leaf shapes of the grouped hashing: one chunk per column (aligned and
+# unaligned elements), two chunks (the pair kernel), 3 chunks (not a power
+# of two: the LDS tree kernel), the quad-tree folds, and 8192 chunks (above
+# 2048: one launch per level).  (kind, n, D, eval_type, batch, row_len).
+GROUPED_ITEMS = [
+    ("iprs", 1 << 12, 32, "binpoly", 1, 1 << 12),   # 1 chunk, aligned
+    ("iprs", 1 << 10, 7, "binpoly", 3, 1 << 10),    # 1 chunk, 56-byte elements
+    ("iprs", 1 << 13, 32, "binpoly", 4, 1 << 12),   # 2 chunks
+    ("iprs", 1 << 14, 32, "binpoly", 3, 1 << 12),   # 3 chunks
+    ("iprs", 1 << 15, 64, "binpoly", 1, 1 << 12),   # 4 chunks
+    ("raa", 1 << 14, 64, "binpoly", 1, 128),        # 32 chunks
+    ("raa", 1 << 16, 64, "binpoly", 1, 8),          # 8192 chunks
+    ("iprs", 1 << 11, 0, "i32", 1, 1 << 11),        # 16-byte elements
+    ("raa_signflip", 1 << 12, 64, "binpoly", 1, 64),
+]
+
+
+def build_rows(zg, items):
+    plan_items, meta = [], []
+    for kind, n, D, et, batch, row_len in items:
+        types = zg.IntTypes() if et == "i32" else zg.BinPolyTypes(D)
+        pp = zg.ZipPlus(types).setup(n, make_code(zg, kind, row_len))
+        plan_items.append((pp, types, batch))
+        meta.append((kind, n, D, et, batch, row_len, pp.num_rows))
+    return plan_items, meta
+
+
+@pytest.mark.parametrize("schedule", [(True, 4), (False, 4), (True, 1), (False, 8), (True, 8)])
+def test_batch_schedules_match_oracle(zg, dev, oracle, schedule):
+    say("schedule", schedule)
+    plan_items, meta = build_rows(zg, GROUPED_ITEMS)
+    plan = zg.BatchCommit(plan_items, graph=True, n_streams=8)
+    plan.set_schedule(*schedule)
+    assert plan.schedule == schedule
+    xs = inputs(meta, 11)
+    want = oracle_roots(oracle, meta, xs)
+    xd = [to_dev(x, dev) for x in xs]
+    for rnd in range(3):  # measure, capture, replay
+        got = plan.commit(xd)
+        assert [c.root for c in got] == want, f"commit {rnd}"
+        eager_noise(zg, dev)
+
+
+def test_batch_many_items_grouped(zg, dev, oracle):
+    # 60 items: more than one group (48 items) per grouped launch.
+    items = []
+    for i in range(60):
+        kind = ("iprs", "raa", "raa_signflip")[i % 3]
+        n = 1 << (10 + i % 4)
+        D = (32, 64, 16)[i % 3]
+        row_len = min(n, 1 << 10) if kind == "iprs" else 32
+        items.append((kind, n, D, "binpoly", 1 + (i % 5 == 0), row_len))
+    plan_items, meta = build_rows(zg, items)
+    plan = zg.BatchCommit(plan_items, graph=True)
+    assert plan.schedule[0] is True
+    xs = inputs(meta, 12)
+    want = oracle_roots(oracle, meta, xs)
+    xd = [to_dev(x, dev) for x in xs]
+    for rnd in range(3):
+        got = plan.commit(xd)
+        assert [c.root for c in got] == want, f"commit {rnd}"
+
+
+def test_batch_schedule_autotune(zg, dev, oracle):
+    plan_items, meta = build_rows(zg, GROUPED_ITEMS[:6])
+    plan = zg.BatchCommit(plan_items, graph=True, autotune=True, n_streams=4)
+    xs = inputs(meta, 13)
+    want = oracle_roots(oracle, meta, xs)
+    xd = [to_dev(x, dev) for x in xs]
+    for rnd in range(3):
+        got = plan.commit(xd)
+        assert [c.root for c in got] == want, f"commit {rnd}"
+    grouped, streams = plan.schedule
+    assert streams in (2, 4, 6) and isinstance(grouped, bool)
+    with pytest.raises(zg.ZipError):
+        plan.set_schedule(True, 99)

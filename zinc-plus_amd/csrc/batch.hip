@@ -4,12 +4,17 @@
 // ZipPlus::commit over its own polys, phase_commit.rs:61-140, with its own
 // code and shape), their hints in one device pool, and a fixed scratch
 // arena.  Commitments are spread over a few branch streams (largest first,
-// each to the least loaded branch) so that small trees overlap on the
-// device.  The first commit of a plan runs eagerly and measures the scratch
-// it needs; later commits either run eagerly from the arena or, with
-// ZP_BATCH_GRAPH, replay one HIP graph of the whole batch (captured again
-// only when the input addresses change).  The roots are gathered into one
-// device buffer and copied to the host once.
+// each to the least loaded branch) so that small items overlap on the
+// device.  The leaf hashing either follows each item's encoder on its
+// branch, or (grouped, the default) runs for all items at once after the
+// branches join: one persistent launch for every item's BLAKE3 chunks, one
+// launch per chunk-tree shape and one per band of Merkle levels.  The first
+// commit of a plan runs eagerly and measures the scratch it needs (with
+// ZP_BATCH_AUTOTUNE it first times the schedules: grouped or per item, 2, 4
+// or 8 branches, and keeps the fastest); later commits either run eagerly
+// from the arena or, with ZP_BATCH_GRAPH, replay one HIP graph of the whole
+// batch (captured again only when the input addresses change).  The roots
+// are gathered into one device buffer and copied to the host once.
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -26,6 +31,9 @@ struct zp_batch {
   std::vector<zp_hint> hints;
   std::vector<uint32_t> order;      // enqueue order (largest first)
   std::vector<uint32_t> branch_of;  // item -> branch stream
+  std::vector<uint64_t> cost;       // codeword bytes per item
+  uint32_t active = 0;              // branch streams in use (<= branch.size())
+  bool grouped = true;              // leaf hashing in grouped launches
   std::vector<hipStream_t> branch;
   std::vector<hipEvent_t> ev_join;
   hipEvent_t ev_fork = nullptr;
@@ -42,6 +50,7 @@ struct zp_batch {
   // A checked code whose overflow check reads a verdict on the host cannot
   // be captured: such a plan replays its commits eagerly.
   bool eager_only = false;
+  bool schedule_set = false;  // tuned, or set by the caller
 };
 
 namespace zpi {
@@ -79,24 +88,62 @@ void destroy_plan(zp_batch* b) {
   delete b;
 }
 
+// Largest first, each to the least loaded of the first `k` branches.
+void assign_branches(zp_batch* b, uint32_t k) {
+  b->active = std::max(1u, std::min<uint32_t>(k, (uint32_t)b->branch.size()));
+  std::vector<uint64_t> load(b->active, 0);
+  b->branch_of.assign(b->n, 0);
+  for (uint32_t i : b->order) {
+    const uint32_t k2 = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
+    b->branch_of[i] = k2;
+    load[k2] += b->cost[i] + 1;
+  }
+}
+
+// Encodes every poly of item i on stream st (the encoding half of
+// commit_enqueue).
+int encode_item(zp_batch* b, uint32_t i, const void* d_evals, hipStream_t st) {
+  const zp_commit_desc& d = b->desc[i];
+  const zp_hint& h = b->hints[i];
+  const uint64_t in_stride =
+      d.num_rows * d.code->row_len * eval_bytes_of(d.eval_type, d.degree_plus_one);
+  for (uint32_t p = 0; p < d.batch; p++)
+    ZP_TRY(encode_rows_variant(d.code, d.eval_type, d.degree_plus_one,
+                               (const uint8_t*)d_evals + p * in_stride, d.num_rows,
+                               h.d_cw + p * h.per_poly, st));
+  return ZP_OK;
+}
+
 // Forks the branches from `s`, enqueues every commitment, joins, gathers.
 int enqueue_all(zp_batch* b, const void* const* d_evals, hipStream_t s) {
   ZP_HIP(hipEventRecord(b->ev_fork, s));
-  for (auto br : b->branch) ZP_HIP(hipStreamWaitEvent(br, b->ev_fork, 0));
+  for (uint32_t k = 0; k < b->active; k++) ZP_HIP(hipStreamWaitEvent(b->branch[k], b->ev_fork, 0));
   int rc = ZP_OK;
   for (uint32_t i : b->order) {
     const zp_commit_desc& d = b->desc[i];
-    rc = commit_enqueue(d.code, d.eval_type, d.degree_plus_one, d_evals[i], d.num_rows, d.batch, &b->hints[i],
-                        b->branch[b->branch_of[i]], false);
+    hipStream_t st = b->branch[b->branch_of[i]];
+    rc = b->grouped ? encode_item(b, i, d_evals[i], st)
+                    : commit_enqueue(d.code, d.eval_type, d.degree_plus_one, d_evals[i],
+                                     d.num_rows, d.batch, &b->hints[i], st, false);
     if (rc != ZP_OK) break;
   }
   // Join every branch even after an error, so that a capture can end.
-  for (size_t k = 0; k < b->branch.size(); k++) {
+  for (uint32_t k = 0; k < b->active; k++) {
     if (hipEventRecord(b->ev_join[k], b->branch[k]) != hipSuccess ||
         hipStreamWaitEvent(s, b->ev_join[k], 0) != hipSuccess)
       rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "branch join failed") : rc;
   }
   if (rc != ZP_OK) return rc;
+  if (b->grouped) {
+    std::vector<LeafJob> jobs(b->n);
+    for (uint32_t i = 0; i < b->n; i++) {
+      const zp_commit_desc& d = b->desc[i];
+      const zp_hint& h = b->hints[i];
+      jobs[i] = {h.d_cw, d.num_rows * d.batch, h.N, (uint32_t)h.elem, h.d_layers,
+                 d.code->tune_ctree_m};
+    }
+    ZP_TRY(launch_grouped_hash(jobs, s));
+  }
   hipLaunchKernelGGL(gather_roots_kernel, dim3((b->n * 8 + 255) / 256), dim3(256), 0, s,
                      b->d_rootptr, b->n, b->d_roots);
   ZP_HIP(hipGetLastError());
@@ -141,6 +188,34 @@ int time_ms(hipStream_t s, int reps, F&& f, float* ms) {
   (void)hipEventDestroy(a);
   if (z) (void)hipEventDestroy(z);
   return rc;
+}
+
+// Times every schedule (grouped or per-item hashing, 2, 4 or 8 branches)
+// eagerly on the plan's own inputs and keeps the fastest.  The runs write
+// the plan's hints like any commit.
+int tune_schedule(zp_batch* b, const void* const* d_evals, hipStream_t s) {
+  float best = 0.f;
+  bool best_grouped = b->grouped;
+  uint32_t best_k = b->active;
+  bool first = true;
+  for (int g = 1; g >= 0; g--)
+    for (uint32_t k : {2u, 4u, 8u}) {
+      if (k > b->branch.size() && k != 2) continue;
+      b->grouped = g == 1;
+      assign_branches(b, k);
+      float ms = 0.f;
+      ZP_TRY(time_ms(s, 2, [&] { return enqueue_all(b, d_evals, s); }, &ms));
+      if (first || ms < best) {
+        best = ms;
+        best_grouped = b->grouped;
+        best_k = b->active;
+        first = false;
+      }
+    }
+  b->grouped = best_grouped;
+  assign_branches(b, best_k);
+  b->schedule_set = true;
+  return ZP_OK;
 }
 
 }  // namespace
@@ -248,7 +323,8 @@ int zp_batch_create(const zp_commit_desc* items, uint32_t n_items, uint32_t flag
   b->hints.resize(n_items);
   int rc = hipGetDevice(&b->device) == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "no device");
   // Hint shapes (validated like zp_commit) and their places in the pool.
-  std::vector<uint64_t> cost(n_items);
+  std::vector<uint64_t>& cost = b->cost;
+  cost.assign(n_items, 0);
   for (uint32_t i = 0; rc == ZP_OK && i < n_items; i++) {
     const zp_commit_desc& d = items[i];
     if (!d.code) {
@@ -290,20 +366,16 @@ int zp_batch_create(const zp_commit_desc* items, uint32_t n_items, uint32_t flag
       rootptr[i] = h.d_layers + (2 * h.N - 2) * 32;
     }
   }
-  // Largest first, each to the least loaded branch.
-  const uint32_t nb = std::max(1u, std::min(n_streams ? n_streams : 4u, n_items));
+  // Largest first, each to the least loaded branch.  The autotuner may pick
+  // up to 8 branches, so a tuned plan creates 8.
+  const uint32_t want = n_streams ? n_streams : 4u;
+  const uint32_t nb = std::max(1u, std::min(std::max(want, (flags & ZP_BATCH_AUTOTUNE) ? 8u : 1u),
+                                            n_items));
   if (rc == ZP_OK) {
     b->order.resize(n_items);
     for (uint32_t i = 0; i < n_items; i++) b->order[i] = i;
     std::stable_sort(b->order.begin(), b->order.end(),
                      [&](uint32_t x, uint32_t y) { return cost[x] > cost[y]; });
-    std::vector<uint64_t> load(nb, 0);
-    b->branch_of.resize(n_items);
-    for (uint32_t i : b->order) {
-      const uint32_t k = (uint32_t)(std::min_element(load.begin(), load.end()) - load.begin());
-      b->branch_of[i] = k;
-      load[k] += cost[i] + 1;
-    }
   }
   for (uint32_t k = 0; rc == ZP_OK && k < nb; k++) {
     hipStream_t st = nullptr;
@@ -317,6 +389,7 @@ int zp_batch_create(const zp_commit_desc* items, uint32_t n_items, uint32_t flag
     else if (rc == ZP_OK)
       b->ev_join.push_back(ev);
   }
+  if (rc == ZP_OK) assign_branches(b, std::min(want, nb));
   if (rc == ZP_OK && (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess ||
                       hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking) != hipSuccess))
     rc = fail(ZP_ERR_DEVICE, "stream/event creation failed");
@@ -360,11 +433,12 @@ int zp_batch_commit(zp_batch* b, const void* const* d_evals, uint8_t* h_roots, z
                                                            std::to_string(i));
   int rc = ZP_OK;
   if (!b->measured) {
+    if ((b->flags & ZP_BATCH_AUTOTUNE) && !b->schedule_set) rc = tune_schedule(b, d_evals, s);
     // First commit: eager, scratch from the cache, sizes counted.
     Arena m;
     m.measuring = true;
     set_arena(&m);
-    rc = enqueue_all(b, d_evals, s);
+    if (rc == ZP_OK) rc = enqueue_all(b, d_evals, s);
     set_arena(nullptr);
     if (rc == ZP_OK && m.used) {  // an arena of size 0 refuses every request
       if (hipMalloc((void**)&b->arena.base, m.used) != hipSuccess) {
@@ -416,6 +490,31 @@ int zp_batch_info(const zp_batch* b, uint64_t* hint_bytes, uint64_t* scratch_byt
   if (hint_bytes) *hint_bytes = b->pool_bytes;
   if (scratch_bytes) *scratch_bytes = b->arena.size;
   if (graph_ready) *graph_ready = b->exec ? 1u : 0u;
+  return ZP_OK;
+}
+
+int zp_batch_schedule(const zp_batch* b, uint32_t* grouped, uint32_t* streams) {
+  if (!b) return fail(ZP_ERR_INVALID_PARAM, "null plan");
+  if (grouped) *grouped = b->grouped ? 1u : 0u;
+  if (streams) *streams = b->active;
+  return ZP_OK;
+}
+
+int zp_batch_set_schedule(zp_batch* b, uint32_t grouped, uint32_t streams) {
+  if (!b) return fail(ZP_ERR_INVALID_PARAM, "null plan");
+  if (grouped > 1 || streams == 0 || streams > b->branch.size())
+    return fail(ZP_ERR_INVALID_PARAM, "grouped is 0 or 1 and streams 1.." +
+                                          std::to_string(b->branch.size()));
+  // The next commit measures the scratch again and captures a new graph.
+  if (hipDeviceSynchronize() != hipSuccess) return fail(ZP_ERR_DEVICE, "synchronize failed");
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  b->exec = nullptr;
+  (void)hipFree(b->arena.base);
+  b->arena = Arena{};
+  b->measured = false;
+  b->grouped = grouped == 1;
+  assign_branches(b, streams);
+  b->schedule_set = true;
   return ZP_OK;
 }
 

@@ -218,6 +218,18 @@ int launch_framed_chunk_cvs(const uint8_t* data, uint64_t entry_bytes, uint64_t 
 // A second stream (with events) per device for overlapping commit stages.
 int aux_stream(hipStream_t* out);
 int launch_merkle_layers(uint8_t* layers, uint64_t n, hipStream_t s);
+// One commitment's leaf hashing for launch_grouped_hash: nrows rows of
+// `width` elements of `elem` bytes; leaves and layers at `layers`.
+struct LeafJob {
+  const uint8_t* rows;
+  uint64_t nrows, width;
+  uint32_t elem;
+  uint8_t* layers;
+  int ctree_m;
+};
+// Leaves and Merkle layers of every job in grouped launches.
+int launch_grouped_hash(const std::vector<LeafJob>& jobs, hipStream_t s);
+uint32_t ctree_fold(uint64_t nchunks, uint64_t width, int ctree_m);
 int launch_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
                         uint32_t batch, const uint64_t* evals, const int64_t* alphas,
                         const int64_t* coeffs, uint64_t* out, void* scratch, hipStream_t s);

@@ -7,6 +7,8 @@
 // specification (7-round compression, 1 KiB chunks, 64-byte blocks, flags
 // CHUNK_START/CHUNK_END/PARENT/ROOT, left subtree = largest power of two of
 // chunks below the total).
+#include <type_traits>
+
 #include "common.h"
 
 namespace zpi {
@@ -97,13 +99,10 @@ __device__ __forceinline__ void load_block_words(const uint8_t* p, uint32_t m[16
 // output is the leaf; otherwise the chunk's chaining value goes to cvs.
 // kAligned: elem is a multiple of 64, so every block is 64 contiguous bytes.
 template <bool kAligned>
-__global__ void __launch_bounds__(256)
-chunk_cv_kernel(const uint8_t* __restrict__ rows, uint64_t nrows, uint64_t width, uint32_t elem,
-                uint64_t nchunks, uint64_t ci_begin, uint64_t ci_end, uint32_t* __restrict__ cvs,
-                uint8_t* __restrict__ leaves) {
-  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint64_t j = idx % width, ci = ci_begin + idx / width;
-  if (ci >= ci_end) return;
+__device__ __forceinline__ void chunk_cv_one(const uint8_t* __restrict__ rows, uint64_t nrows,
+                                             uint64_t width, uint32_t elem, uint64_t nchunks,
+                                             uint64_t j, uint64_t ci, uint32_t* __restrict__ cvs,
+                                             uint8_t* __restrict__ leaves) {
   const uint64_t total = nrows * elem;
   const uint64_t c0 = ci * 1024;
   const uint64_t clen = min((uint64_t)1024, total - c0);
@@ -155,6 +154,17 @@ chunk_cv_kernel(const uint8_t* __restrict__ rows, uint64_t nrows, uint64_t width
     dst[0] = make_uint4(cv[0], cv[1], cv[2], cv[3]);
     dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
   }
+}
+
+template <bool kAligned>
+__global__ void __launch_bounds__(256)
+chunk_cv_kernel(const uint8_t* __restrict__ rows, uint64_t nrows, uint64_t width, uint32_t elem,
+                uint64_t nchunks, uint64_t ci_begin, uint64_t ci_end, uint32_t* __restrict__ cvs,
+                uint8_t* __restrict__ leaves) {
+  const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t j = idx % width, ci = ci_begin + idx / width;
+  if (ci >= ci_end) return;
+  chunk_cv_one<kAligned>(rows, nrows, width, elem, nchunks, j, ci, cvs, leaves);
 }
 
 // One level of a column's chunk tree: pairs (2i, 2i+1) merge, an odd last
@@ -319,12 +329,12 @@ __device__ __forceinline__ void parent_quad(const uint32_t* msg, uint32_t q, boo
 // with one node uses the root merge.  Every level is one compression
 // deep, so a quad per node cuts the latency of the 16 levels of a 2^16-leaf
 // tree about threefold against a thread per node.
-__global__ void __launch_bounds__(512)
-merkle_quad_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32_t levels) {
-  __shared__ uint32_t buf[256 * 8];
+__device__ __forceinline__ void merkle_quad_block(uint8_t* __restrict__ layers, uint64_t n,
+                                                  uint32_t L0, uint32_t levels, uint64_t block,
+                                                  uint32_t* buf) {
   const uint32_t t = threadIdx.x, quad = t >> 2, q = t & 3;
   const uint64_t cnt0 = n >> L0;
-  const uint64_t base = (uint64_t)blockIdx.x * 256;
+  const uint64_t base = block * 256;
   const uint32_t m = (uint32_t)min((uint64_t)256, cnt0 - base);
   const uint32_t* src = (const uint32_t*)(layers + 32 * (2 * n - 2 * cnt0));
   for (uint32_t i = t; i < m * 8; i += 512) buf[i] = src[base * 8 + i];
@@ -348,6 +358,12 @@ merkle_quad_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32
   }
 }
 
+__global__ void __launch_bounds__(512)
+merkle_quad_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32_t levels) {
+  __shared__ uint32_t buf[256 * 8];
+  merkle_quad_block(layers, n, L0, levels, blockIdx.x, buf);
+}
+
 // Chunk trees of columns of nchunks = C*M chunks (C in 4..256 and M in
 // 1..8, powers of two), C lanes per column and no chunk values in LDS.
 // Lane l folds chunks lM .. lM + M - 1, a perfect subtree, in registers:
@@ -359,15 +375,15 @@ merkle_quad_kernel(uint8_t* __restrict__ layers, uint64_t n, uint32_t L0, uint32
 // place after a barrier.  Columns past the last fold column 0 and store
 // nothing, so every quad stays whole for the DPP rotations.
 template <int M>
-__global__ void __launch_bounds__(256)
-cv_quad_tree_kernel(const uint32_t* __restrict__ cvs, uint64_t width, uint32_t C,
-                    uint8_t* __restrict__ leaves) {
+__device__ __forceinline__ void cv_quad_tree_block(const uint32_t* __restrict__ cvs,
+                                                   uint64_t width, uint32_t C,
+                                                   uint8_t* __restrict__ leaves, uint64_t block,
+                                                   uint32_t* nodes) {
   static_assert(M >= 1 && M <= 8 && (M & (M - 1)) == 0, "M is 1, 2, 4 or 8");
   constexpr int kDepth = M >= 8 ? 4 : M >= 4 ? 3 : M >= 2 ? 2 : 1;
-  __shared__ uint32_t nodes[256 * 8];
   const uint32_t tid = threadIdx.x, q = tid & 3;
   const uint32_t l = tid & (C - 1), col = tid / C;
-  const uint64_t j = (uint64_t)blockIdx.x * (256 / C) + col;
+  const uint64_t j = block * (256 / C) + col;
   const bool live = j < width;
   const uint64_t jc = live ? j : 0;
   uint32_t stk[kDepth][8];
@@ -414,6 +430,14 @@ cv_quad_tree_kernel(const uint32_t* __restrict__ cvs, uint64_t width, uint32_t C
     dst[q] = lo[0];
     dst[4 + q] = hi[0];
   }
+}
+
+template <int M>
+__global__ void __launch_bounds__(256)
+cv_quad_tree_kernel(const uint32_t* __restrict__ cvs, uint64_t width, uint32_t C,
+                    uint8_t* __restrict__ leaves) {
+  __shared__ uint32_t nodes[256 * 8];
+  cv_quad_tree_block<M>(cvs, width, C, leaves, blockIdx.x, nodes);
 }
 
 // Merkle layers (merkle.rs:172-221) for n = 2^h leaves stored at layers[0..n).
@@ -512,6 +536,112 @@ stream_chunk_cv_kernel(const uint8_t* __restrict__ stream, uint64_t c0, uint64_t
   dst[1] = make_uint4(cv[4], cv[5], cv[6], cv[7]);
 }
 
+// Grouped launches for many commitments at once (batched commits).  Each
+// launch carries its items by value (no device tables, so a captured graph
+// holds no copies); an item owns a contiguous range of the launch's tiles or
+// blocks, and a block finds its item by binary search over the range starts.
+constexpr uint32_t kGroupMax = 48;
+
+struct LeafTask {
+  const uint8_t* rows;  // codeword rows (nrows x width elements of elem bytes)
+  uint32_t* cvs;        // chunk chaining values (nchunks > 1)
+  uint8_t* leaves;
+  uint64_t nrows;
+  uint32_t width, elem, nchunks, tile0;
+};
+struct LeafGroup {
+  LeafTask t[kGroupMax];
+  uint32_t n, tiles;
+};
+static_assert(sizeof(LeafGroup) < 4000, "kernel argument limit");
+
+struct TreeTask {
+  const uint32_t* cvs;
+  uint8_t* leaves;
+  uint32_t width, C, block0, pad;
+};
+struct TreeGroup {
+  TreeTask t[kGroupMax];
+  uint32_t n, blocks;
+};
+
+struct MerkleTask {
+  uint8_t* layers;
+  uint64_t n;
+  uint32_t levels, block0, blocks, pad;
+};
+struct MerkleGroup {
+  MerkleTask t[kGroupMax];
+  uint32_t n, blocks, L0;
+};
+
+// The item whose range [start, next start) holds `x` (starts increase).
+template <class G>
+__device__ __forceinline__ uint32_t find_item(const G& g, uint32_t x, uint32_t (*start)(const G&, uint32_t)) {
+  uint32_t lo = 0, hi = g.n;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (start(g, mid) <= x) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+__device__ __forceinline__ uint32_t leaf_start(const LeafGroup& g, uint32_t i) { return g.t[i].tile0; }
+__device__ __forceinline__ uint32_t tree_start(const TreeGroup& g, uint32_t i) { return g.t[i].block0; }
+__device__ __forceinline__ uint32_t merkle_start(const MerkleGroup& g, uint32_t i) { return g.t[i].block0; }
+
+// Chunk chaining values (or single-chunk leaves) of every item: a
+// persistent grid walks the tiles of 256 (column, chunk) tasks, largest
+// items first, so the small items' tails fill in behind the large ones.
+__global__ void __launch_bounds__(256) grouped_chunk_cv_kernel(LeafGroup g) {
+  for (uint32_t tile = blockIdx.x; tile < g.tiles; tile += gridDim.x) {
+    const LeafTask& it = g.t[find_item(g, tile, leaf_start)];
+    const uint64_t idx = (uint64_t)(tile - it.tile0) * 256 + threadIdx.x;
+    const uint64_t j = idx % it.width, ci = idx / it.width;
+    if (ci >= it.nchunks) continue;
+    if ((it.elem & 63) == 0)
+      chunk_cv_one<true>(it.rows, it.nrows, it.width, it.elem, it.nchunks, j, ci, it.cvs,
+                         it.leaves);
+    else
+      chunk_cv_one<false>(it.rows, it.nrows, it.width, it.elem, it.nchunks, j, ci, it.cvs,
+                          it.leaves);
+  }
+}
+
+// Chunk trees of items with nchunks = C * M (the quad kernel's shapes).
+template <int M>
+__global__ void __launch_bounds__(256) grouped_cv_quad_tree_kernel(TreeGroup g) {
+  __shared__ uint32_t nodes[256 * 8];
+  const TreeTask& it = g.t[find_item(g, blockIdx.x, tree_start)];
+  cv_quad_tree_block<M>(it.cvs, it.width, it.C, it.leaves, blockIdx.x - it.block0, nodes);
+}
+
+// Two-chunk columns: the root merge of the pair.  Thread per column.
+__global__ void __launch_bounds__(256) grouped_cv_pair_kernel(TreeGroup g) {
+  const TreeTask& it = g.t[find_item(g, blockIdx.x, tree_start)];
+  const uint64_t j = (uint64_t)(blockIdx.x - it.block0) * 256 + threadIdx.x;
+  if (j >= it.width) return;
+  uint32_t l[8], r[8];
+  const uint4* a = (const uint4*)(it.cvs + j * 8);
+  const uint4* b = (const uint4*)(it.cvs + ((uint64_t)it.width + j) * 8);
+  const uint4 a0 = a[0], a1 = a[1], b0 = b[0], b1 = b[1];
+  l[0] = a0.x; l[1] = a0.y; l[2] = a0.z; l[3] = a0.w;
+  l[4] = a1.x; l[5] = a1.y; l[6] = a1.z; l[7] = a1.w;
+  r[0] = b0.x; r[1] = b0.y; r[2] = b0.z; r[3] = b0.w;
+  r[4] = b1.x; r[5] = b1.y; r[6] = b1.z; r[7] = b1.w;
+  parent_cv(l, r, true);
+  uint32_t* dst = (uint32_t*)(it.leaves + j * 32);
+#pragma unroll
+  for (int i = 0; i < 8; i++) dst[i] = r[i];
+}
+
+// Up to 8 Merkle layers above layer L0 of every item that has one.
+__global__ void __launch_bounds__(512) grouped_merkle_quad_kernel(MerkleGroup g) {
+  __shared__ uint32_t buf[256 * 8];
+  const MerkleTask& it = g.t[find_item(g, blockIdx.x, merkle_start)];
+  merkle_quad_block(it.layers, it.n, g.L0, it.levels, blockIdx.x - it.block0, buf);
+}
+
 }  // namespace
 
 int launch_framed_chunk_cvs(const uint8_t* data, uint64_t entry_bytes, uint64_t q0, uint64_t c0,
@@ -585,20 +715,26 @@ std::vector<int> ctree_fold_candidates(uint64_t nchunks) {
   return c;
 }
 
+// Chunks per lane of the quad chunk-tree kernel for this shape, 0 where it
+// does not apply.  Up to 8 chunks per lane, fewer while that leaves under
+// 2^18 lanes in all (4 waves per SIMD) and a column stays within 256 lanes.
+// More lanes per column add quad levels: at 16 chunks x 2^16 columns, 4
+// lanes of 4 chunks take 26.5 us and 8 lanes of 2 take 31.4 us.  A tuned
+// fold (ctree_m) replaces the rule when the shape admits it.
+uint32_t ctree_fold(uint64_t nchunks, uint64_t width, int ctree_m) {
+  if (g_force_ctree == 1 || !pow2(nchunks) || nchunks < 4 || nchunks > 2048) return 0;
+  uint64_t M = std::min<uint64_t>(8, nchunks / 4);
+  while (M > 1 && width * (nchunks / M) < (1u << 18) && nchunks / M < 256) M /= 2;
+  if (ctree_m > 0)
+    for (int c : ctree_fold_candidates(nchunks))
+      if (c == ctree_m) M = (uint64_t)c;
+  return (uint32_t)M;
+}
+
 int launch_cv_merge(uint32_t* cvs, uint32_t* tmp, uint64_t nchunks, uint64_t width,
                     uint8_t* leaves, hipStream_t s, int ctree_m) {
   if (nchunks <= 1) return ZP_OK;
-  if (g_force_ctree != 1 && pow2(nchunks) && nchunks >= 4 && nchunks <= 2048) {
-    // Up to 8 chunks per lane, fewer while that leaves under 2^18 lanes in
-    // all (4 waves per SIMD) and a column stays within 256 lanes.  More
-    // lanes per column add quad levels: at 16 chunks x 2^16 columns, 4 lanes
-    // of 4 chunks take 26.5 us and 8 lanes of 2 take 31.4 us.  A tuned fold
-    // (ctree_m) replaces the rule when the shape admits it.
-    uint64_t M = std::min<uint64_t>(8, nchunks / 4);
-    while (M > 1 && width * (nchunks / M) < (1u << 18) && nchunks / M < 256) M /= 2;
-    if (ctree_m > 0)
-      for (int c : ctree_fold_candidates(nchunks))
-        if (c == ctree_m) M = (uint64_t)c;
+  if (const uint64_t M = ctree_fold(nchunks, width, ctree_m)) {
     const uint32_t C = (uint32_t)(nchunks / M);
     switch (M) {
       case 1: return launch_cv_quad_tree<1>(cvs, width, C, leaves, s);
@@ -646,6 +782,193 @@ int launch_leaf_hash(const uint8_t* rows, uint64_t nrows, uint64_t width, uint32
   if (rc == ZP_OK) rc = launch_cv_merge(cv[0], cv[1], nchunks, width, leaves, s, ctree_m);
   dev_free(cv[0], s);
   dev_free(cv[1], s);
+  return rc;
+}
+
+namespace {
+
+// Resident 256-thread blocks of the grouped chunk kernel on this device:
+// the persistent grid's size.
+uint32_t grouped_cv_capacity() {
+  static int cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 1024;
+  if (!cached[dev]) {
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, grouped_chunk_cv_kernel, 256, 0) !=
+            hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        per_cu <= 0 || cus <= 0) {
+      (void)hipGetLastError();
+      return 1024;
+    }
+    cached[dev] = per_cu * cus;
+  }
+  return (uint32_t)cached[dev];
+}
+
+template <class G, class T>
+int launch_groups(const std::vector<T>& tasks, uint32_t (*units)(const T&), G proto,
+                  void (*launch)(const G&, uint32_t, hipStream_t), hipStream_t s) {
+  for (size_t i0 = 0; i0 < tasks.size(); i0 += kGroupMax) {
+    G g = proto;
+    g.n = 0;
+    uint32_t total = 0;
+    for (size_t i = i0; i < std::min(tasks.size(), i0 + kGroupMax); i++) {
+      g.t[g.n] = tasks[i];
+      uint32_t* start = nullptr;
+      if constexpr (std::is_same<T, LeafTask>::value) start = &g.t[g.n].tile0;
+      else start = &g.t[g.n].block0;
+      *start = total;
+      total += units(tasks[i]);
+      g.n++;
+    }
+    if (total == 0) continue;
+    if constexpr (std::is_same<T, LeafTask>::value) g.tiles = total;
+    else g.blocks = total;
+    launch(g, total, s);
+    ZP_HIP(hipGetLastError());
+  }
+  return ZP_OK;
+}
+
+}  // namespace
+
+// Leaves and Merkle layers of many commitments in grouped launches: one
+// persistent launch for every item's chunk chaining values, one launch per
+// chunk-tree shape, one per band of 8 Merkle levels.  Shapes the grouped
+// kernels do not take (chunk counts that are not a power of two, or above
+// 2048, and option merkle_tree = 1) run through the per-item launchers on
+// the same stream.
+int launch_grouped_hash(const std::vector<LeafJob>& jobs, hipStream_t s) {
+  struct Plan {
+    uint64_t nchunks = 0, cv_off = 0, tmp_off = 0;
+    uint32_t M = 0;
+  };
+  std::vector<Plan> plan(jobs.size());
+  uint64_t bytes = 0;
+  auto take = [&](uint64_t b) {
+    const uint64_t off = bytes;
+    bytes += (b + 255) / 256 * 256;
+    return off;
+  };
+  for (size_t i = 0; i < jobs.size(); i++) {
+    const LeafJob& jb = jobs[i];
+    if (jb.elem % 4 != 0) return fail(ZP_ERR_SHAPE, "element size must be a multiple of 4 bytes");
+    if (jb.width == 0) continue;
+    if (jb.width >= (1ull << 32) || jb.nrows >= (1ull << 40))
+      return fail(ZP_ERR_SHAPE, "grouped hashing: shape too large");
+    Plan& p = plan[i];
+    p.nchunks = leaf_chunks(jb.nrows, jb.elem);
+    if (p.nchunks > 1) {
+      p.cv_off = take(p.nchunks * jb.width * 32);
+      p.M = ctree_fold(p.nchunks, jb.width, jb.ctree_m);
+      if (!p.M && p.nchunks != 2) p.tmp_off = take(((p.nchunks + 1) / 2) * jb.width * 32);
+    }
+  }
+  uint8_t* scratch = nullptr;
+  if (bytes) ZP_TRY(dev_alloc((void**)&scratch, bytes, s));
+  int rc = ZP_OK;
+  // Chunk chaining values, largest items first.
+  std::vector<size_t> order;
+  for (size_t i = 0; i < jobs.size(); i++)
+    if (jobs[i].width) order.push_back(i);
+  auto tiles_of = [&](size_t i) { return (plan[i].nchunks * jobs[i].width + 255) / 256; };
+  std::stable_sort(order.begin(), order.end(),
+                   [&](size_t a, size_t b) { return tiles_of(a) > tiles_of(b); });
+  std::vector<LeafTask> leaf;
+  for (size_t i : order) {
+    const LeafJob& jb = jobs[i];
+    if (tiles_of(i) >= (1ull << 31)) {
+      rc = fail(ZP_ERR_SHAPE, "grouped hashing: too many chunks");
+      break;
+    }
+    leaf.push_back({jb.rows, plan[i].nchunks > 1 ? (uint32_t*)(scratch + plan[i].cv_off) : nullptr,
+                    jb.layers, jb.nrows, (uint32_t)jb.width, jb.elem, (uint32_t)plan[i].nchunks,
+                    0});
+  }
+  if (rc == ZP_OK)
+    rc = launch_groups<LeafGroup, LeafTask>(
+        leaf,
+        [](const LeafTask& t) { return (uint32_t)(((uint64_t)t.nchunks * t.width + 255) / 256); },
+        LeafGroup{},
+        [](const LeafGroup& g, uint32_t total, hipStream_t st) {
+          const uint32_t grid = std::min(total, grouped_cv_capacity());
+          hipLaunchKernelGGL(grouped_chunk_cv_kernel, dim3(grid), dim3(256), 0, st, g);
+        },
+        s);
+  // Chunk trees.
+  std::vector<TreeTask> quad[4], pair;
+  for (size_t i = 0; rc == ZP_OK && i < jobs.size(); i++) {
+    const LeafJob& jb = jobs[i];
+    const Plan& p = plan[i];
+    if (!jb.width || p.nchunks <= 1) continue;
+    uint32_t* cvs = (uint32_t*)(scratch + p.cv_off);
+    if (p.M) {
+      const uint32_t C = (uint32_t)(p.nchunks / p.M);
+      quad[__builtin_ctz(p.M)].push_back({cvs, jb.layers, (uint32_t)jb.width, C, 0, 0});
+    } else if (p.nchunks == 2) {
+      pair.push_back({cvs, jb.layers, (uint32_t)jb.width, 2, 0, 0});
+    } else {
+      rc = launch_cv_merge(cvs, (uint32_t*)(scratch + p.tmp_off), p.nchunks, jb.width, jb.layers,
+                           s, jb.ctree_m);
+    }
+  }
+  auto quad_blocks = [](const TreeTask& t) {
+    const uint32_t per = 256 / t.C;
+    return (t.width + per - 1) / per;
+  };
+  if (rc == ZP_OK)
+    rc = launch_groups<TreeGroup, TreeTask>(quad[0], quad_blocks, TreeGroup{},
+        [](const TreeGroup& g, uint32_t total, hipStream_t st) {
+          hipLaunchKernelGGL(grouped_cv_quad_tree_kernel<1>, dim3(total), dim3(256), 0, st, g);
+        }, s);
+  if (rc == ZP_OK)
+    rc = launch_groups<TreeGroup, TreeTask>(quad[1], quad_blocks, TreeGroup{},
+        [](const TreeGroup& g, uint32_t total, hipStream_t st) {
+          hipLaunchKernelGGL(grouped_cv_quad_tree_kernel<2>, dim3(total), dim3(256), 0, st, g);
+        }, s);
+  if (rc == ZP_OK)
+    rc = launch_groups<TreeGroup, TreeTask>(quad[2], quad_blocks, TreeGroup{},
+        [](const TreeGroup& g, uint32_t total, hipStream_t st) {
+          hipLaunchKernelGGL(grouped_cv_quad_tree_kernel<4>, dim3(total), dim3(256), 0, st, g);
+        }, s);
+  if (rc == ZP_OK)
+    rc = launch_groups<TreeGroup, TreeTask>(quad[3], quad_blocks, TreeGroup{},
+        [](const TreeGroup& g, uint32_t total, hipStream_t st) {
+          hipLaunchKernelGGL(grouped_cv_quad_tree_kernel<8>, dim3(total), dim3(256), 0, st, g);
+        }, s);
+  if (rc == ZP_OK)
+    rc = launch_groups<TreeGroup, TreeTask>(
+        pair, [](const TreeTask& t) { return (t.width + 255) / 256; }, TreeGroup{},
+        [](const TreeGroup& g, uint32_t total, hipStream_t st) {
+          hipLaunchKernelGGL(grouped_cv_pair_kernel, dim3(total), dim3(256), 0, st, g);
+        }, s);
+  // Merkle layers, 8 levels per band.
+  if (g_force_tree == 1) {
+    for (size_t i = 0; rc == ZP_OK && i < jobs.size(); i++)
+      if (jobs[i].width) rc = launch_merkle_layers(jobs[i].layers, jobs[i].width, s);
+  } else {
+    for (uint32_t L0 = 0; rc == ZP_OK; L0 += 8) {
+      std::vector<MerkleTask> mt;
+      for (const LeafJob& jb : jobs) {
+        const uint64_t cnt = L0 < 64 ? jb.width >> L0 : 0;
+        if (cnt <= 1) continue;
+        const uint32_t lg = 63 - (uint32_t)__builtin_clzll(cnt);
+        mt.push_back({jb.layers, jb.width, std::min<uint32_t>(8, lg), 0,
+                      (uint32_t)((cnt + 255) / 256), 0});
+      }
+      if (mt.empty()) break;
+      MerkleGroup proto{};
+      proto.L0 = L0;
+      rc = launch_groups<MerkleGroup, MerkleTask>(
+          mt, [](const MerkleTask& t) { return t.blocks; }, proto,
+          [](const MerkleGroup& g, uint32_t total, hipStream_t st) {
+            hipLaunchKernelGGL(grouped_merkle_quad_kernel, dim3(total), dim3(512), 0, st, g);
+          }, s);
+    }
+  }
+  if (scratch) dev_free(scratch, s);
   return rc;
 }
 

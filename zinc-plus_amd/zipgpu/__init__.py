@@ -728,6 +728,18 @@ class BatchCommit:
                                        ctypes.byref(g)))
         return {"hint_bytes": hb.value, "scratch_bytes": sb.value, "graph_ready": bool(g.value)}
 
+    @property
+    def schedule(self):
+        """(grouped, streams): whether the leaf hashing of all items runs in
+        grouped launches after the encoders, and the branch streams in use."""
+        g, k = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self._lib.zp_batch_schedule(self._h, ctypes.byref(g), ctypes.byref(k)))
+        return bool(g.value), k.value
+
+    def set_schedule(self, grouped, streams):
+        """Fixes the schedule; the next commit re-measures its scratch."""
+        _check(self._lib.zp_batch_set_schedule(self._h, 1 if grouped else 0, streams))
+
 
 class CommitServer:
     """Continuous-batching commit/open service on one device (zp_server_*).

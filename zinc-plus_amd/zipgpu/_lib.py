@@ -99,6 +99,8 @@ SIGNATURES = {
     "zp_batch_hint": (_c_void_p, [_c_void_p, _u32]),
     "zp_batch_info": (_int, [_c_void_p, ctypes.POINTER(_u64), ctypes.POINTER(_u64),
                              ctypes.POINTER(_u32)]),
+    "zp_batch_schedule": (_int, [_c_void_p, ctypes.POINTER(_u32), ctypes.POINTER(_u32)]),
+    "zp_batch_set_schedule": (_int, [_c_void_p, _u32, _u32]),
     "zp_batch_destroy": (None, [_c_void_p]),
     "zp_autotune": (_int, [_c_void_p, _int, _u32, _u64, _u32, _c_void_p]),
     "zp_code_tuning": (_int, [_c_void_p, ctypes.POINTER(_i32), ctypes.POINTER(_i32),
