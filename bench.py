@@ -355,13 +355,22 @@ def leg_batched(torch, zipgpu, synth, dev):
     plan = zipgpu.BatchCommit(items, graph=True, autotune=True, n_streams=8)
     roots = [c.root for c in plan.commit(xs)]
     plan_ms = wall_ms(torch, lambda: plan.commit(xs), 5, 1)
+    grouped, streams = plan.schedule
     tuned_plain_ms = wall_ms(torch, plain, 3, 1)
-    # The same graph on one branch stream: every item's device work back to
-    # back with no host work between them (the serial device time).
-    serial = zipgpu.BatchCommit(items, graph=True, autotune=False, n_streams=1)
-    serial.commit(xs)
-    serial_ms = wall_ms(torch, lambda: serial.commit(xs), 5, 1)
-    del serial
+    # Each schedule on its own plan (graph replays): grouped leaf hashing
+    # after the encoders, or each item's hashing on its branch stream.
+    schedules = {}
+    for g in (True, False):
+        for k in (1, 4, 8):
+            p2 = zipgpu.BatchCommit(items, graph=True, autotune=False, n_streams=8)
+            p2.set_schedule(g, k)
+            p2.commit(xs)
+            schedules[f"{'grouped' if g else 'per_item'}_{k}_streams_ms"] = wall_ms(
+                torch, lambda: p2.commit(xs), 5, 1)
+            del p2
+    # One branch stream with per-item hashing: every item's device work back
+    # to back with no host work between them (the serial device time).
+    serial_ms = schedules["per_item_1_streams_ms"]
     # The plan's roots equal the plain path's.
     same = roots == [zipgpu.ZipPlus(t).commit(pp, [x])[1].root
                      for (pp, t), x in zip(items, xs)]
@@ -374,7 +383,9 @@ def leg_batched(torch, zipgpu, synth, dev):
             "serial_graph_1_stream_ms": serial_ms, "batched_vs_serial": serial_ms / plan_ms,
             "batched_evals_per_s": total / plan_ms * 1e3, "batched_vs_plain": plain_ms / plan_ms,
             "roots_equal_plain": same, "hint_bytes": info["hint_bytes"],
-            "scratch_bytes": info["scratch_bytes"]}
+            "scratch_bytes": info["scratch_bytes"],
+            "tuned_schedule": {"grouped_hashing": grouped, "branch_streams": streams},
+            "schedules": schedules}
 
 
 def served_requests(zipgpu, synth, seed, count, lo, hi):
@@ -590,7 +601,9 @@ def main():
 
     def enc(s):
         code.encode_rows(x, pp.num_rows, types, out=out, stream=s)
-    enc_ms = event_ms(torch, enc, stream, max(5, args.steps))
+    # Steady clocks, as for the RAA legs: 20 launches first (the host-input
+    # legs just before leave the device idle between copies).
+    enc_ms = event_ms(torch, enc, stream, max(40, args.steps), warmup=20)
     zipgpu.set_option("iprs_encoder", 1)
     try:
         valu_ms = event_ms(torch, enc, stream, 5)
