@@ -46,7 +46,8 @@ struct zp_code {
   int32_t* d_tw[zpi::kMaxIprsDepth] = {nullptr, nullptr, nullptr};  // [bd*8^k][7][8]
   // MFMA encoder tables (depth 1 or 2): int8 slices of the base matrix in
   // the v_mfma_i32_16x16x64_i8 operand layout, and stage twiddles as double.
-  void* d_afrag = nullptr;    // [bd/16][ksteps][3 slices][64 lanes][16 bytes]
+  void* d_afrag = nullptr;    // [bd/16][ksteps][3 slices][64 lanes][16 bytes]; r-block rb
+                              // holds rows r = rb + (bd/16) m, m = lane % 16
   // Stage twiddles as double in butterfly order: entry (j-1)*4 + t of a
   // position holds the coefficient of input j in output t < 4 (outputs
   // t + 4 follow by symmetry); 32 entries per position, 28 used.

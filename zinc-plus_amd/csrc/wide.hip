@@ -371,7 +371,7 @@ wide_mfma_kernel(const wv4i* __restrict__ afrag, const wv4i* __restrict__ bfrag,
   uint32_t* dst[4];
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const uint32_t r = rb * 16 + 4 * (lane >> 4) + v;
+    const uint32_t r = rb + (bd / 16) * (4 * (lane >> 4) + v);  // the A table's rows
     dst[v] = (uint32_t*)(out + lay.out_at(rp, c * bd + r));
   }
   // The B fragments of the next four digits are loaded while the current
