@@ -379,7 +379,8 @@ void zp_server_destroy(zp_server* server);
  *   comb[col] = sum_b sum_j s_j * w'_b[j*row_len + col]
  * in CombR = Int<comb_limbs> (two's complement mod 2^(64*comb_limbs)).
  * d_alphas: batch x D i128; d_coeffs: num_rows i128 (the reference uses [1]
- * when num_rows == 1); d_out: row_len x comb_limbs u64. */
+ * when num_rows == 1); d_out: row_len x comb_limbs u64.  Asynchronous on
+ * `stream`: d_out is complete once the stream passes this call. */
 int zp_combined_row(uint32_t degree_plus_one, uint32_t comb_limbs, uint64_t num_rows,
                     uint64_t row_len, uint32_t batch, const uint64_t* d_evals,
                     const int64_t* d_alphas, const int64_t* d_coeffs, uint64_t* d_out,

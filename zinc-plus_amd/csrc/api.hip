@@ -616,10 +616,10 @@ int zp_combined_row(uint32_t D, uint32_t L, uint64_t num_rows, uint64_t row_len,
     return fail(ZP_ERR_INVALID_PARAM, "null buffer");
   void* scratch = nullptr;
   ZP_TRY(dev_alloc(&scratch, combined_row_scratch_bytes(D, row_len, batch, num_rows), s));
-  int rc = launch_combined_row(D, L, num_rows, row_len, batch, d_evals, d_alphas, d_coeffs, d_out,
-                               scratch, s);
-  if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
-    rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
+  // Stream-ordered: the scratch goes back to this stream's cache, where only
+  // later work on the same stream can take it.
+  const int rc = launch_combined_row(D, L, num_rows, row_len, batch, d_evals, d_alphas, d_coeffs,
+                                     d_out, scratch, s);
   dev_free(scratch, s);
   return rc;
 }

@@ -862,7 +862,10 @@ int launch_grouped_hash(const std::vector<LeafJob>& jobs, hipStream_t s) {
     p.nchunks = leaf_chunks(jb.nrows, jb.elem);
     if (p.nchunks > 1) {
       p.cv_off = take(p.nchunks * jb.width * 32);
-      p.M = ctree_fold(p.nchunks, jb.width, jb.ctree_m);
+      // One launch holds every item's columns, so no item needs its lanes
+      // spread for occupancy: the largest fold (fewest quad levels, whose
+      // merges cost about 1.6x the lane-operations of a folded one).
+      p.M = ctree_fold(p.nchunks, ~0ull >> 8, -1);
       if (!p.M && p.nchunks != 2) p.tmp_off = take(((p.nchunks + 1) / 2) * jb.width * 32);
     }
   }

@@ -656,10 +656,15 @@ class ZipPlus:
         if alphas.numel() != t.shape[0] * D * 2 or coeffs.numel() != pp.num_rows * 2:
             raise ShapeError(L.ZP_ERR_SHAPE, "alphas/coeffs have the wrong length")
         out = torch.empty((row_len, Lm), dtype=torch.int64, device=t.device)
+        al, co = alphas.contiguous(), coeffs.contiguous()
         _check(L.load().zp_combined_row(D, Lm, pp.num_rows, row_len, t.shape[0], _dev_ptr(t),
-                                        _dev_ptr(alphas.contiguous()),
-                                        _dev_ptr(coeffs.contiguous()), _dev_ptr(out),
+                                        _dev_ptr(al), _dev_ptr(co), _dev_ptr(out),
                                         _stream(stream)))
+        # The call is stream-ordered: tensors made here (a stacked batch,
+        # contiguous copies) stay reserved until `stream` has used them.
+        if stream is not None:
+            for x in (t, al, co, out):
+                x.record_stream(stream)
         return out
 
 
