@@ -73,61 +73,65 @@ constexpr uint64_t kMagicBits = 0x4338000000000000ull;
 constexpr double kMagicHi = kMagic - (double)(kMagicBits >> 32);
 static_assert((kMagicBits & 0xffffffffull) == 0, "bias must be a multiple of 2^32");
 
-// planes[(((row*NC + c)*(D/16) + p/16)*W + w)*16 + p%16] holds the 16 bits of plane p for
-// k = 16w .. 16w+15 in "spread" order: bit i = 4v + b of the group sits at
-// position 8b + v, so (word >> v) & 0x01010101 is operand register v of the
-// MFMA B fragment (bytes b = 0..3 for k = 16w + 4v + b) in two instructions.
-// One thread per (row, c, w, 16 planes; D is a multiple of 16 on this
-// path): a lane writes its 16 words as four 16-byte stores, and the chunks
-// whose octet reversals are adjacent follow on neighbouring lanes, so one
-// load instruction reads adjacent evals.  Each group of 16 planes has its
-// own run of words, so the main kernel's loads read whole cache lines: with
-// the planes of a word interleaved, a workgroup read half of every line it
-// touched (the other plane group's half), and the extra line traffic
-// through the CU's L1 slowed the stores of the workgroup beside it
-// (0.268 -> 0.257 ms per 2^20-eval encode).  With 4 planes per lane, eight lanes loaded the same
-// 16 evals and the pre-pass was bound by load issue (12 -> ~8 µs at 2^20).
+// Plane words: planes[((((row*NC + c)*(D/16) + p/16)*4 + g)*16 + p%16)*S2 + s2]
+// (S2 = ceil(S/2)) holds, for plane p, the bits the MFMA lane group g takes
+// in k-steps 2 s2 and 2 s2 + 1 (k = 64 s + 16 g + i, i < 16), in "spread"
+// order: bit i = 4v + b of k-step 2 s2 + h sits at position 8b + v + 4h, so
+// (word >> (4h + v)) & 0x01010101 is operand register v of that k-step's
+// MFMA B fragment (bytes b = 0..3) in two instructions.  Two k-steps share a
+// word and a lane's words of one tile are adjacent, so the main kernel reads
+// a tile with one 8-byte load: 32 KiB of plane words per workgroup instead
+// of 64 KiB in 4-byte loads (0.245-0.256 -> 0.238-0.241 ms per 2^20-eval
+// encode).  Each group of 16 planes has its own run of words, so the loads
+// read whole cache lines.  One thread per (row, c, g, s2, 16 planes; D is a
+// multiple of 16 on this path), and the chunks whose octet reversals are
+// adjacent follow on neighbouring lanes, so one load instruction reads
+// adjacent evals.
 constexpr uint32_t kPlanesPerLane = 16;
 template <int NC>
 __global__ void __launch_bounds__(256)
 bitplane_kernel(const uint64_t* __restrict__ evals, uint32_t* __restrict__ planes,
-                uint32_t row_len, uint32_t bl, uint32_t W, uint32_t D, uint32_t depth,
+                uint32_t row_len, uint32_t bl, uint32_t S, uint32_t D, uint32_t depth,
                 uint64_t rows) {
   constexpr uint32_t PPL = kPlanesPerLane;
-  const uint32_t PQ = D / PPL;
+  const uint32_t PQ = D / PPL, S2 = (S + 1) / 2;
   const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= rows * NC * W * PQ) return;
+  if (idx >= rows * NC * 4 * S2 * PQ) return;
   const uint32_t pq = (uint32_t)(idx % PQ);
   uint64_t rest = idx / PQ;
   const uint32_t h = (uint32_t)(rest % 8);
   rest /= 8;
   const uint32_t a = (uint32_t)(rest % (NC / 8));
   rest /= NC / 8;
-  const uint32_t w = (uint32_t)(rest % W);
-  const uint64_t row = rest / W;
+  const uint32_t s2 = (uint32_t)(rest % S2);
+  rest /= S2;
+  const uint32_t g = (uint32_t)(rest % 4);
+  const uint64_t row = rest / 4;
   // Depth 2: chunk c = a + 8h has octet reversal 8a + h.  Depth 1: c = h.
   const uint32_t c = NC == 8 ? h : a + 8 * h;
   const uint64_t* x = evals + row * row_len + octrev_d(c, depth);
-  // Every load is issued (clamped to the last k) and the values past the
-  // base length are masked: no branch keeps the 16 loads from overlapping.
-  uint64_t xs[16];
-#pragma unroll
-  for (int i = 0; i < 16; i++) xs[i] = x[(uint64_t)NC * min(16 * w + i, bl - 1)];
   uint32_t v[PPL];
 #pragma unroll
   for (int q = 0; q < (int)PPL; q++) v[q] = 0;
 #pragma unroll
-  for (int i = 0; i < 16; i++) {
-    const uint32_t k = 16 * w + i;
-    const uint32_t bits = k < bl ? (uint32_t)(xs[i] >> (PPL * pq)) : 0u;
-    const uint32_t pos = 8 * (i & 3) + (i >> 2);
+  for (int hh = 0; hh < 2; hh++) {
+    // k-step 2 s2 + hh, lane group g: k = 64 s + 16 g + i.
+    const uint32_t k0 = 64 * (2 * s2 + hh) + 16 * g;
+    uint64_t xs[16];
 #pragma unroll
-    for (int q = 0; q < (int)PPL; q++) v[q] |= ((bits >> q) & 1u) << pos;
+    for (int i = 0; i < 16; i++) xs[i] = x[(uint64_t)NC * min(k0 + i, bl - 1)];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+      const uint32_t k = k0 + i;
+      const uint32_t bits = k < bl ? (uint32_t)(xs[i] >> (PPL * pq)) : 0u;
+      const uint32_t pos = 8 * (i & 3) + (i >> 2) + 4 * hh;
+#pragma unroll
+      for (int q = 0; q < (int)PPL; q++) v[q] |= ((bits >> q) & 1u) << pos;
+    }
   }
-  uint4* dst = (uint4*)(planes + (((row * NC + c) * PQ + pq) * W + w) * PPL);
+  uint32_t* dst = planes + ((((row * NC + c) * PQ + pq) * 4 + g) * PPL) * S2 + s2;
 #pragma unroll
-  for (int q = 0; q < (int)PPL / 4; q++)
-    dst[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+  for (int q = 0; q < (int)PPL; q++) dst[q * S2] = v[q];
 }
 
 // Radix-8 butterflies by symmetry.  Output t of a butterfly is
@@ -210,7 +214,6 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   // Base sums [c][r_local][p], from the matrix cores to the stage threads.
   constexpr uint32_t kBsWords = NC * 16 * 16;
   __shared__ int32_t Bs[kBsWords];
-  constexpr uint32_t W = 4 * S;
   constexpr uint32_t kTiles = NC / 4;  // chunks per wave
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t g = lane >> 4, pcol = lane & 15;
@@ -242,14 +245,21 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
   // (phase 2 has rl = tid / 16 and p = tid % 16).
   // One row's plane words span NC*W*D*4 <= 256 KiB: 32-bit offsets from the
   // (uniform) row base.
-  const char* prow = (const char*)(planes + row * NC * W * D);
-  const uint32_t poff = (((wave * npg + pg) * W + g) * 16 + pcol) * 4;
-  uint32_t pw[kTiles][S];
+  constexpr uint32_t S2 = (S + 1) / 2;
+  const char* prow = (const char*)(planes + row * NC * 4 * S2 * D);
+  const uint32_t poff = ((((wave * npg + pg) * 4 + g) * 16 + pcol) * S2) * 4;
+  uint32_t pw[kTiles][S2];
 #pragma unroll
-  for (uint32_t i = 0; i < kTiles; i++)
-#pragma unroll
-    for (int s = 0; s < S; s++)
-      pw[i][s] = *(const uint32_t*)(prow + (poff + 4 * i * W * D * 4 + 4 * s * 64));
+  for (uint32_t i = 0; i < kTiles; i++) {
+    const char* src = prow + (poff + 4 * i * npg * 4 * 16 * S2 * 4);
+    if constexpr (S2 == 2) {
+      const uint2 w2 = *(const uint2*)src;
+      pw[i][0] = w2.x;
+      pw[i][1] = w2.y;
+    } else {
+      pw[i][0] = *(const uint32_t*)src;
+    }
+  }
   v4i A[S][3];
 #pragma unroll
   for (int s = 0; s < S; s++)
@@ -269,7 +279,7 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
     v4i acc0 = {0, 0, 0, 0}, acc1 = {0, 0, 0, 0}, acc2 = {0, 0, 0, 0};
 #pragma unroll
     for (int s = 0; s < S; s++) {
-      const uint32_t w = pw[i][s];
+      const uint32_t w = pw[i][s / 2] >> (4 * (s & 1));
       v4i bf;
       bf[0] = (int)(w & 0x01010101u);
       bf[1] = (int)((w >> 1) & 0x01010101u);
@@ -418,7 +428,7 @@ int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32
   const uint64_t max_rows = std::max<uint64_t>(1, 0x7fffffffull / per_row);
   for (uint64_t r0 = 0; r0 < rows; r0 += max_rows) {
     const uint64_t nr = std::min(max_rows, rows - r0);
-    const uint32_t* pl = planes + r0 * NC * 4 * S * Dp;
+    const uint32_t* pl = planes + r0 * NC * 4 * ((S + 1) / 2) * Dp;
     if (D % 16 == 0)
       hipLaunchKernelGGL((iprs_mfma_kernel<NC, S, false>), dim3((uint32_t)(nr * per_row)),
                          dim3(256), 0, s, pl, (const v4i*)cd->d_afrag, cd->d_tw0f, cd->d_tw1f,
@@ -435,14 +445,14 @@ int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32
 template <int NC>
 int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D, int64_t* out,
               hipStream_t s) {
-  const uint32_t S = cd->mfma_ksteps, W = 4 * S;
+  const uint32_t S = cd->mfma_ksteps, S2 = (S + 1) / 2;
   const uint32_t Dp = (D + 15) / 16 * 16;  // planes above D are computed, not stored
-  const uint64_t bytes = rows * NC * W * Dp * sizeof(uint32_t);
+  const uint64_t bytes = rows * NC * 4 * S2 * Dp * sizeof(uint32_t);
   uint32_t* planes = nullptr;
   ZP_TRY(dev_alloc((void**)&planes, bytes, s));
-  const uint64_t threads = rows * NC * W * (Dp / kPlanesPerLane);
+  const uint64_t threads = rows * NC * 4 * S2 * (Dp / kPlanesPerLane);
   hipLaunchKernelGGL((bitplane_kernel<NC>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,
-                     s, evals, planes, (uint32_t)cd->row_len, cd->bl, W, Dp, cd->depth, rows);
+                     s, evals, planes, (uint32_t)cd->row_len, cd->bl, S, Dp, cd->depth, rows);
   int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "bitplane launch");
   if (rc == ZP_OK) {
     switch (S) {
@@ -461,7 +471,7 @@ int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t 
 
 uint64_t iprs_mfma_scratch_bytes(const zp_code* cd, uint64_t rows, uint32_t D) {
   const uint64_t NC = cd->depth == 1 ? 8 : 64;
-  return rows * NC * 4 * cd->mfma_ksteps * ((D + 15) / 16 * 16) * sizeof(uint32_t);
+  return rows * NC * 4 * ((cd->mfma_ksteps + 1) / 2) * ((D + 15) / 16 * 16) * sizeof(uint32_t);
 }
 
 bool iprs_mfma_eligible(const zp_code* cd, uint32_t D) {
