@@ -307,3 +307,21 @@ def test_batch_schedule_autotune(zg, dev, oracle):
     assert streams in (2, 4, 6) and isinstance(grouped, bool)
     with pytest.raises(zg.ZipError):
         plan.set_schedule(True, 99)
+
+
+def test_batch_schedule_change_after_capture(zg, dev, oracle):
+    # A plan that has measured its arena and captured its graph takes a new
+    # schedule: the next commit measures again and captures anew.
+    plan_items, meta = build_rows(zg, GROUPED_ITEMS[:5])
+    plan = zg.BatchCommit(plan_items, graph=True, n_streams=4)
+    xs = inputs(meta, 14)
+    want = oracle_roots(oracle, meta, xs)
+    xd = [to_dev(x, dev) for x in xs]
+    for sched in [None, (False, 2), (True, 3), (False, 1)]:
+        if sched is not None:
+            plan.set_schedule(*sched)
+            assert plan.schedule == sched
+        for rnd in range(3):
+            got = plan.commit(xd)
+            assert [c.root for c in got] == want, f"schedule {sched}, commit {rnd}"
+        assert plan.info()["graph_ready"]

@@ -94,6 +94,7 @@ void free_code(zp_code* c) {
   (void)hipFree(c->d_ftab);
   for (auto* p : c->d_tw) (void)hipFree(p);
   (void)hipFree(c->d_afrag);
+  (void)hipFree(c->d_afrag_c);
   (void)hipFree(c->d_tw0f);
   (void)hipFree(c->d_tw1f);
   (void)hipFree(c->d_q0f);
@@ -389,8 +390,12 @@ int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_c
       const uint32_t ksteps = (bl + 63) / 64;
       if (rc == ZP_OK && (c->depth == 1 || c->depth == 2) && bd >= 16 && ksteps <= 4) {
         std::vector<int8_t> af((uint64_t)(bd / 16) * ksteps * 3 * 64 * 16);
-        iprs_afrag(bd, bl, ksteps, f.data(), af.data());
+        iprs_afrag(bd, bl, ksteps, f.data(), af.data(), true);
         rc = upload(af, (int8_t**)&c->d_afrag);
+        if (rc == ZP_OK) {
+          iprs_afrag(bd, bl, ksteps, f.data(), af.data(), false);
+          rc = upload(af, (int8_t**)&c->d_afrag_c);
+        }
         for (uint32_t k = 0; rc == ZP_OK && k < c->depth; k++) {
           std::vector<double> tf;
           std::vector<uint16_t> qm;

@@ -47,7 +47,8 @@ struct zp_code {
   // MFMA encoder tables (depth 1 or 2): int8 slices of the base matrix in
   // the v_mfma_i32_16x16x64_i8 operand layout, and stage twiddles as double.
   void* d_afrag = nullptr;    // [bd/16][ksteps][3 slices][64 lanes][16 bytes]; r-block rb
-                              // holds rows r = rb + (bd/16) m, m = lane % 16
+                              // holds rows r = rb + (bd/16) m, m = lane % 16 (binary rows)
+  void* d_afrag_c = nullptr;  // the same with rows r = 16 rb + m (encode_wide, step 0)
   // Stage twiddles as double in butterfly order: entry (j-1)*4 + t of a
   // position holds the coefficient of input j in output t < 4 (outputs
   // t + 4 follow by symmetry); 32 entries per position, 28 used.
@@ -138,7 +139,8 @@ uint64_t iprs_optimal_depth(uint64_t row_len);
 void iprs_ftab(uint32_t bd, int32_t* out);
 void iprs_twiddles(uint32_t bd, uint64_t N, uint32_t depth, uint32_t stage, int32_t* out);
 void raa_perm(uint64_t n, uint64_t seed, uint32_t* out);
-void iprs_afrag(uint32_t bd, uint32_t bl, uint32_t ksteps, const int32_t* ftab, int8_t* out);
+void iprs_afrag(uint32_t bd, uint32_t bl, uint32_t ksteps, const int32_t* ftab, int8_t* out,
+                bool spread);
 
 // Kernel launchers (asynchronous on `s`).
 int launch_iprs_binary(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,

@@ -104,17 +104,19 @@ void iprs_twiddles(uint32_t bd, uint64_t N, uint32_t depth, uint32_t stage, int3
     }
 }
 
-// Base-matrix slices for the MFMA encoder.  f = s0 + 128*s1 + 16384*s2 with
+// Base-matrix slices for the MFMA encoders.  f = s0 + 128*s1 + 16384*s2 with
 // s0 = f & 127, s1 = (f >> 7) & 127, s2 = f >> 14 (in [-3, 1]).  Lane l of
-// the A operand holds row r = rb + (bd/16)*(l % 16) and bytes for
-// k = 64*s + 16*(l / 16) + byte; columns k >= bl are zero.
-void iprs_afrag(uint32_t bd, uint32_t bl, uint32_t ksteps, const int32_t* ftab, int8_t* out) {
+// the A operand holds row r = rb + (bd/16)*(l % 16) (spread) or
+// r = 16*rb + l % 16 (consecutive) and bytes for k = 64*s + 16*(l / 16) +
+// byte; columns k >= bl are zero.
+void iprs_afrag(uint32_t bd, uint32_t bl, uint32_t ksteps, const int32_t* ftab, int8_t* out,
+                bool spread) {
   const uint32_t nrb = bd / 16;
   for (uint32_t rb = 0; rb < nrb; rb++)
     for (uint32_t s = 0; s < ksteps; s++)
       for (uint32_t q = 0; q < 3; q++)
         for (uint32_t lane = 0; lane < 64; lane++) {
-          const uint32_t r = rb + nrb * (lane % 16);  // rows nrb apart
+          const uint32_t r = spread ? rb + nrb * (lane % 16) : 16 * rb + lane % 16;
           int8_t* dst = out + ((((uint64_t)rb * ksteps + s) * 3 + q) * 64 + lane) * 16;
           for (uint32_t b = 0; b < 16; b++) {
             const uint32_t k = 64 * s + 16 * (lane / 16) + b;

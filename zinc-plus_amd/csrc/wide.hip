@@ -23,7 +23,7 @@ namespace zpi {
 // of its base matrix (depth 1 or 2, base length <= 256), unless the
 // per-element kernel is forced.
 bool wide_mfma_eligible(const zp_code* cd) {
-  return cd->d_afrag != nullptr && (cd->depth == 1 || cd->depth == 2) && g_force_wide != 1;
+  return cd->d_afrag_c != nullptr && (cd->depth == 1 || cd->depth == 2) && g_force_wide != 1;
 }
 
 namespace {
@@ -371,7 +371,7 @@ wide_mfma_kernel(const wv4i* __restrict__ afrag, const wv4i* __restrict__ bfrag,
   uint32_t* dst[4];
 #pragma unroll
   for (int v = 0; v < 4; v++) {
-    const uint32_t r = rb + (bd / 16) * (4 * (lane >> 4) + v);  // the A table's rows
+    const uint32_t r = rb * 16 + 4 * (lane >> 4) + v;  // consecutive rows (d_afrag_c)
     dst[v] = (uint32_t*)(out + lay.out_at(rp, c * bd + r));
   }
   // The B fragments of the next four digits are loaded while the current
@@ -443,7 +443,7 @@ int launch_wide_mfma_s(const zp_code* cd, const uint64_t* in, uint64_t seqs,
                        0, s, in, frag, lay, cd->bl, cd->depth, nchunks, (uint32_t)S, nd, ncols,
                        total);
     hipLaunchKernelGGL((wide_mfma_kernel<L, S>), dim3((uint32_t)blocks), dim3(256), 0, s,
-                       (const wv4i*)cd->d_afrag, (const wv4i*)frag, out, lay, cd->bd, nd, nchunks,
+                       (const wv4i*)cd->d_afrag_c, (const wv4i*)frag, out, lay, cd->bd, nd, nchunks,
                        ncols, nct, nrb);
     if (hipGetLastError() != hipSuccess) rc = fail(ZP_ERR_DEVICE, "launch failed");
   }
