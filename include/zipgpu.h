@@ -82,8 +82,10 @@ const char* zp_last_error(void);
 /* Library-wide overrides of the kernel variants (no reference counterpart).
  * -1 (the initial value) leaves the choice to each code's tuned variant
  * (zp_autotune) or, for an untuned code, the default (variant 0 / 1 below).
- *   "iprs_encoder": 0 = matrix-core encoder where eligible (default),
- *                   1 = VALU encoder for every shape.
+ *   "iprs_encoder": 0 = matrix-core encoder where eligible, an r-block's
+ *                   16 base rows bd/16 apart (default), 1 = VALU encoder
+ *                   for every shape, 2 = matrix-core encoder with 16
+ *                   consecutive base rows per r-block.
  *   "raa_kernel":   0 = full-row kernel where eligible (default),
  *                   1 = plane groups (8 planes per workgroup at codeword
  *                       length 8192), 2 = plane groups (4 planes at 8192).
@@ -291,7 +293,8 @@ void zp_batch_destroy(zp_batch* plan);
  * groups), the chunk-tree fold of the leaf hashing (1, 2, 4 or 8 chunk
  * chaining values per lane before the quad merges), and the encode/hash
  * overlap.  IPRS binary codes take the matrix-core encoder wherever it
- * applies.  Must not run concurrently with other uses of the code. */
+ * applies, with its r-block rows spread or consecutive as timed.  Must not
+ * run concurrently with other uses of the code. */
 int zp_autotune(zp_code* code, int eval_type, uint32_t degree_plus_one, uint64_t num_rows,
                 uint32_t batch, zp_stream stream);
 /* The tuned variants (-1 = not tuned), numbered as in zp_set_option. */

@@ -53,8 +53,8 @@ def test_iprs_binary_ignores_bits_above_degree(zg, dev, oracle):
     (256, 1, 32), (2048, 1, 64), (1 << 13, 2, 32), (1 << 14, 2, 64), (1 << 12, 2, 16),
     (1 << 14, 2, 20), (256, 1, 8), (2048, 1, 40), (1 << 13, 2, 1), (1 << 12, 2, 63)])
 def test_iprs_device_encoders_agree(zg, dev, oracle, row_len, depth, D):
-    # The matrix-core encoder and the VALU encoder are separate device paths;
-    # both must equal the oracle.  D not a multiple of 16 pads the plane
+    # The matrix-core encoder (its r-block rows spread or consecutive) and
+    # the VALU encoder are separate device paths; all must equal the oracle.  D not a multiple of 16 pads the plane
     # words and skips the stores of the extra planes.
     say("encoders", row_len, depth, D)
     rows = 2
@@ -65,12 +65,15 @@ def test_iprs_device_encoders_agree(zg, dev, oracle, row_len, depth, D):
     try:
         zg.set_option("iprs_encoder", 0)
         fast = code.encode_rows(xd, rows, types).cpu().numpy()
+        zg.set_option("iprs_encoder", 2)
+        fast_c = code.encode_rows(xd, rows, types).cpu().numpy()
         zg.set_option("iprs_encoder", 1)
         valu = code.encode_rows(xd, rows, types).cpu().numpy()
     finally:
         zg.set_option("iprs_encoder", -1)
     want = oracle.iprs_encode_binary(x.view(np.uint64), rows, row_len, depth, D)
-    assert np.array_equal(fast, want) and np.array_equal(valu, want)
+    assert np.array_equal(fast, want) and np.array_equal(fast_c, want)
+    assert np.array_equal(valu, want)
 
 
 @pytest.mark.parametrize("row_len,depth,rows", [

@@ -160,7 +160,7 @@ int lc_budget_check(uint32_t rep, uint64_t row_len, int eval_type, uint32_t D, u
 }
 
 bool iprs_uses_mfma(const zp_code* c, uint32_t D) {
-  return iprs_mfma_eligible(c, D) && pick(g_force_iprs, c->tune_iprs, 0) == 0;
+  return iprs_mfma_eligible(c, D) && pick(g_force_iprs, c->tune_iprs, 0) != 1;
 }
 
 // Encodes with the variant in force for this code (types already checked).
@@ -181,7 +181,8 @@ int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void*
   if (c->kind == ZP_CODE_IPRS) {
     if (eval_type == ZP_EVAL_BINPOLY) {
       if (iprs_uses_mfma(c, D))
-        return launch_iprs_binary_mfma(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
+        return launch_iprs_binary_mfma(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s,
+                                       pick(g_force_iprs, c->tune_iprs, 0) != 2);
       return launch_iprs_binary(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s);
     }
     if (wide_mfma_eligible(c))
@@ -318,7 +319,7 @@ int zp_set_option(const char* name, int64_t value) {
     int* var;
     int64_t max;
   };
-  const Opt opts[] = {{"iprs_encoder", &g_force_iprs, 1},
+  const Opt opts[] = {{"iprs_encoder", &g_force_iprs, 2},
                       {"raa_kernel", &g_force_raa, 2},
                       {"commit_overlap", &g_force_overlap, 1},
                       {"wide_base", &g_force_wide, 1},

@@ -274,8 +274,12 @@ int zp_autotune(zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uin
     // One encoder (the Int<L> kernels): nothing to tune.
   } else if (code->kind == ZP_CODE_IPRS) {
     // The matrix-core encoder wherever it applies (the VALU encoder is 14x
-    // slower at every such shape); otherwise the VALU encoder.
-    code->tune_iprs = (eval_type == ZP_EVAL_BINPOLY && iprs_mfma_eligible(code, D)) ? 0 : 1;
+    // slower at every such shape), its r-blocks' rows spread (0) or
+    // consecutive (2) as timed; otherwise the VALU encoder.
+    if (eval_type == ZP_EVAL_BINPOLY && iprs_mfma_eligible(code, D))
+      tune(&code->tune_iprs, {0, 2}, encode_all);
+    else
+      code->tune_iprs = 1;
   } else {
     std::vector<int> c;
     if (raa_full_row_eligible(code, D)) c.push_back(0);

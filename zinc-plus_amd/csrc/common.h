@@ -63,7 +63,7 @@ struct zp_code {
   uint32_t* d_perm1 = nullptr;
   uint32_t* d_perm2 = nullptr;
   // Kernel variants picked by the autotuner (-1 = not tuned: defaults).
-  mutable int tune_iprs = -1;     // 0 matrix-core encoder, 1 VALU encoder
+  mutable int tune_iprs = -1;     // 0 matrix cores (rows spread), 1 VALU, 2 matrix cores (rows consecutive)
   mutable int tune_raa = -1;      // 0 full-row kernel, 1 plane groups (8), 2 plane groups (4)
   mutable int tune_overlap = -1;  // commit: 1 overlap encode and hash on two streams, 0 not
   // Chunks folded per lane by the chunk-tree kernel of this code's commits
@@ -149,8 +149,10 @@ bool iprs_mfma_eligible(const zp_code* c, uint32_t D);
 // Bytes of bit-plane words the matrix-core encoder takes from the cache for
 // `rows` rows (one allocation per launch).
 uint64_t iprs_mfma_scratch_bytes(const zp_code* c, uint64_t rows, uint32_t D);
+// spread: r-block rows bd/16 apart (d_afrag), else 16 consecutive rows
+// (d_afrag_c); iprs_encoder variants 0 and 2.
 int launch_iprs_binary_mfma(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
-                            int64_t* out, hipStream_t s);
+                            int64_t* out, hipStream_t s, bool spread = true);
 int launch_iprs_i32(const zp_code* c, const int32_t* evals, uint64_t rows, void* out,
                     hipStream_t s);
 bool wide_mfma_eligible(const zp_code* c);

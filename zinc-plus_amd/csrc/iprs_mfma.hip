@@ -210,7 +210,7 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
                  const double* __restrict__ tw0, const double* __restrict__ tw1,
                  const uint16_t* __restrict__ q0, const uint16_t* __restrict__ q1,
                  int64_t* __restrict__ out, uint32_t bd, uint32_t D, uint32_t Dout, uint32_t nrb,
-                 uint32_t npg, uint64_t row0) {
+                 uint32_t npg, uint64_t row0, uint32_t rmul, uint32_t rstep) {
   // Base sums [c][r_local][p], from the matrix cores to the stage threads.
   constexpr uint32_t kBsWords = NC * 16 * 16;
   __shared__ int32_t Bs[kBsWords];
@@ -266,8 +266,10 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
 #pragma unroll
     for (int q = 0; q < 3; q++) A[s][q] = afrag[(((uint64_t)rb * S + s) * 3 + q) * 64 + lane];
   double t0r[2];
-  load_spread(t0r, tw0 + (uint64_t)(rb + nrb * (tid >> 4)) * 32, tid & 15);
-  const uint32_t m0 = q0[rb + nrb * (tid >> 4)];
+  // Row m of r-block rb is r = rb rmul + m rstep: (1, bd/16) spreads the
+  // block's rows, (16, 1) takes 16 consecutive rows (the A table matches).
+  load_spread(t0r, tw0 + (uint64_t)(rb * rmul + rstep * (tid >> 4)) * 32, tid & 15);
+  const uint32_t m0 = q0[rb * rmul + rstep * (tid >> 4)];
   // Keep the loads here: left to the scheduler they sink to their first
   // uses and their latency is exposed there.
   __builtin_amdgcn_sched_barrier(0);
@@ -302,7 +304,7 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
 
   // Phase 2: every radix-8 stage of one (r, p) column per thread.
   const uint32_t rl = tid >> 4, p = tid & 15;
-  const uint32_t r = rb + nrb * rl;
+  const uint32_t r = rb * rmul + rstep * rl;
   const uint64_t N = (uint64_t)bd * NC;
   // One row's codewords span N*D*8 <= 32 MiB, so 32-bit element offsets from
   // the row base address every output.
@@ -421,9 +423,11 @@ iprs_mfma_kernel(const uint32_t* __restrict__ planes, const v4i* __restrict__ af
 
 template <int NC, int S>
 int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32_t D,
-                int64_t* out, hipStream_t s) {
+                int64_t* out, hipStream_t s, bool spread) {
   const uint32_t Dp = (D + 15) / 16 * 16;  // plane-word layout
   const uint32_t nrb = cd->bd / 16, npg = Dp / 16;
+  const v4i* afrag = (const v4i*)(spread ? cd->d_afrag : cd->d_afrag_c);
+  const uint32_t rmul = spread ? 1u : 16u, rstep = spread ? nrb : 1u;
   const uint64_t per_row = (uint64_t)nrb * npg;
   const uint64_t max_rows = std::max<uint64_t>(1, 0x7fffffffull / per_row);
   for (uint64_t r0 = 0; r0 < rows; r0 += max_rows) {
@@ -431,12 +435,12 @@ int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32
     const uint32_t* pl = planes + r0 * NC * 4 * ((S + 1) / 2) * Dp;
     if (D % 16 == 0)
       hipLaunchKernelGGL((iprs_mfma_kernel<NC, S, false>), dim3((uint32_t)(nr * per_row)),
-                         dim3(256), 0, s, pl, (const v4i*)cd->d_afrag, cd->d_tw0f, cd->d_tw1f,
-                         cd->d_q0f, cd->d_q1f, out, cd->bd, Dp, D, nrb, npg, r0);
+                         dim3(256), 0, s, pl, afrag, cd->d_tw0f, cd->d_tw1f, cd->d_q0f,
+                         cd->d_q1f, out, cd->bd, Dp, D, nrb, npg, r0, rmul, rstep);
     else
       hipLaunchKernelGGL((iprs_mfma_kernel<NC, S, true>), dim3((uint32_t)(nr * per_row)),
-                         dim3(256), 0, s, pl, (const v4i*)cd->d_afrag, cd->d_tw0f, cd->d_tw1f,
-                         cd->d_q0f, cd->d_q1f, out, cd->bd, Dp, D, nrb, npg, r0);
+                         dim3(256), 0, s, pl, afrag, cd->d_tw0f, cd->d_tw1f, cd->d_q0f,
+                         cd->d_q1f, out, cd->bd, Dp, D, nrb, npg, r0, rmul, rstep);
     ZP_HIP(hipGetLastError());
   }
   return ZP_OK;
@@ -444,7 +448,7 @@ int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32
 
 template <int NC>
 int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D, int64_t* out,
-              hipStream_t s) {
+              hipStream_t s, bool spread) {
   const uint32_t S = cd->mfma_ksteps, S2 = (S + 1) / 2;
   const uint32_t Dp = (D + 15) / 16 * 16;  // planes above D are computed, not stored
   const uint64_t bytes = rows * NC * 4 * S2 * Dp * sizeof(uint32_t);
@@ -456,10 +460,10 @@ int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t 
   int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "bitplane launch");
   if (rc == ZP_OK) {
     switch (S) {
-      case 1: rc = launch_main<NC, 1>(cd, planes, rows, D, out, s); break;
-      case 2: rc = launch_main<NC, 2>(cd, planes, rows, D, out, s); break;
-      case 3: rc = launch_main<NC, 3>(cd, planes, rows, D, out, s); break;
-      case 4: rc = launch_main<NC, 4>(cd, planes, rows, D, out, s); break;
+      case 1: rc = launch_main<NC, 1>(cd, planes, rows, D, out, s, spread); break;
+      case 2: rc = launch_main<NC, 2>(cd, planes, rows, D, out, s, spread); break;
+      case 3: rc = launch_main<NC, 3>(cd, planes, rows, D, out, s, spread); break;
+      case 4: rc = launch_main<NC, 4>(cd, planes, rows, D, out, s, spread); break;
       default: rc = fail(ZP_ERR_INVALID_PARAM, "unsupported base length for the MFMA encoder");
     }
   }
@@ -480,10 +484,10 @@ bool iprs_mfma_eligible(const zp_code* cd, uint32_t D) {
 }
 
 int launch_iprs_binary_mfma(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
-                            int64_t* out, hipStream_t s) {
+                            int64_t* out, hipStream_t s, bool spread) {
   if (rows == 0) return ZP_OK;
-  if (cd->depth == 1) return launch_nc<8>(cd, evals, rows, D, out, s);
-  return launch_nc<64>(cd, evals, rows, D, out, s);
+  if (cd->depth == 1) return launch_nc<8>(cd, evals, rows, D, out, s, spread);
+  return launch_nc<64>(cd, evals, rows, D, out, s, spread);
 }
 
 }  // namespace zpi

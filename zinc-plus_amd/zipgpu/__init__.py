@@ -83,7 +83,8 @@ def lib():
 def set_option(name, value):
     """zp_set_option: library-wide kernel-variant overrides, -1 = automatic.
 
-    "iprs_encoder" (0 matrix-core, 1 VALU), "raa_kernel" (0 full-row,
+    "iprs_encoder" (0 matrix-core with r-block rows spread, 1 VALU, 2
+    matrix-core with consecutive rows), "raa_kernel" (0 full-row,
     1 / 2 plane groups), "commit_overlap" (1 overlap encode and hash, 0 not:
     the default), "wide_base" (Int<L>, i64 and i32 IPRS base layer: 0
     matrix cores where the code has the int8 tables, 1 per-element), "merkle_tree"
