@@ -19,6 +19,7 @@
 // the coefficients s, the column indices, and the proof bytes.
 #include <algorithm>
 #include <cstring>
+#include <new>
 #include <vector>
 
 #include "common.h"
@@ -31,6 +32,8 @@ uint64_t ts_squeeze_idx(zp_transcript* t, uint64_t cap);
 void ts_write_field_elements(zp_transcript* t, const uint64_t mod[4], const Fp* v, size_t n);
 void ts_put(zp_transcript* t, const void* p, size_t n);
 void ts_reserve(zp_transcript* t, size_t more);
+uint8_t* ts_grow_pinned(zp_transcript* t, size_t more);
+void ts_unput(zp_transcript* t, size_t n);
 
 namespace {
 
@@ -666,12 +669,30 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
     if (hipGetLastError() != hipSuccess) return done(fail(ZP_ERR_DEVICE, "launch failed"));
   }
   // Proof bytes: the combined row (write_const_many), then the openings.
-  // The openings come back in pieces, each followed by an event, so the
-  // host appends piece k to the stream while later pieces are in flight.
-  uint8_t* o_comb = pin + up2;
-  uint8_t* o_open = o_comb + row_len * L * 8;
   const uint64_t comb_bytes = row_len * L * 8;
   const uint64_t open_total = (uint64_t)num_openings * open_bytes;
+  // With pinned stream storage the copies land in the stream itself.
+  uint8_t* direct = nullptr;
+  try {
+    direct = ts_grow_pinned(t, comb_bytes + open_total);
+  } catch (const std::bad_alloc&) {
+    return done(fail(ZP_ERR_OOM, "proof stream allocation failed"));
+  }
+  if (direct) {
+    if (hipMemcpyAsync(direct, d_comb, comb_bytes, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        (open_total && hipMemcpyAsync(direct + comb_bytes, d_open, open_total,
+                                      hipMemcpyDeviceToHost, s) != hipSuccess) ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      ts_unput(t, comb_bytes + open_total);
+      return done(fail(ZP_ERR_DEVICE, "proof copy failed"));
+    }
+    return done(ZP_OK);
+  }
+  // Otherwise the openings come back in pieces through the pinned staging
+  // buffer, each followed by an event, so the host appends piece k to the
+  // stream while later pieces are in flight.
+  uint8_t* o_comb = pin + up2;
+  uint8_t* o_open = o_comb + row_len * L * 8;
   constexpr uint64_t kPiece = 16ull << 20;
   const uint64_t npieces = (open_total + kPiece - 1) / kPiece;
   std::vector<hipEvent_t> pev(npieces, nullptr);

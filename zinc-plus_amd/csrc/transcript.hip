@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -426,14 +427,78 @@ void par_memcpy(void* dst, const void* src, size_t n) {
   for (auto& x : th) x.join();
 }
 
-// The proof stream's bytes.  Small streams grow with realloc.  From 4 MiB
-// on (a proof with column openings reaches tens of MiB) the storage is
-// 2 MiB-aligned and advised for transparent huge pages: the first write of
-// a fresh 77 MB stream costs ~19,000 page faults with 4 KiB pages.
+// Large stream storage: pinned host buffers (hipHostMalloc), kept for reuse
+// when a stream lets go of them.  A proof with column openings reaches tens
+// of MiB; fresh pages cost a fault and a kernel zeroing each on first touch
+// (a 77 MB stream: ~19,000 faults with 4 KiB pages), and pinned storage
+// lets the prover copy the proof's device parts straight into the stream.
+// At most kCacheBytes stay cached; the rest is released.
+class PinnedStore {
+ public:
+  // A buffer of at least `want` bytes (rounded to 2 MiB), or null when
+  // pinned memory is not available.
+  static uint8_t* acquire(size_t want, size_t* cap) {
+    const size_t c = (want + kGranule - 1) & ~(kGranule - 1);
+    {
+      std::lock_guard<std::mutex> g(mu());
+      auto& fl = free_list();
+      size_t best = fl.size();
+      for (size_t i = 0; i < fl.size(); i++)
+        if (fl[i].second >= c && (best == fl.size() || fl[i].second < fl[best].second)) best = i;
+      if (best < fl.size()) {
+        const auto e = fl[best];
+        fl.erase(fl.begin() + (long)best);
+        cached() -= e.second;
+        *cap = e.second;
+        return (uint8_t*)e.first;
+      }
+    }
+    void* q = nullptr;
+    if (hipHostMalloc(&q, c, hipHostMallocDefault) != hipSuccess || !q) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+    *cap = c;
+    return (uint8_t*)q;
+  }
+  static void release(uint8_t* p, size_t cap) {
+    if (!p) return;
+    {
+      std::lock_guard<std::mutex> g(mu());
+      if (cached() + cap <= kCacheBytes) {
+        free_list().push_back({p, cap});
+        cached() += cap;
+        return;
+      }
+    }
+    (void)hipHostFree(p);
+  }
+
+ private:
+  static constexpr size_t kGranule = 2u << 20, kCacheBytes = 1ull << 30;
+  // Never destroyed: a stream released during process exit still finds them.
+  static std::mutex& mu() {
+    static std::mutex* m = new std::mutex;
+    return *m;
+  }
+  static std::vector<std::pair<void*, size_t>>& free_list() {
+    static auto* v = new std::vector<std::pair<void*, size_t>>;
+    return *v;
+  }
+  static size_t& cached() {
+    static size_t b = 0;
+    return b;
+  }
+};
+
+// The proof stream's bytes.  Small streams grow with realloc; from 4 MiB on
+// the storage comes from PinnedStore (or, without pinned memory, 2 MiB-
+// aligned pageable memory advised for transparent huge pages).
 struct ByteBuf {
   static constexpr size_t kHuge = 2u << 20, kBig = 4u << 20;
   uint8_t* p = nullptr;
   size_t n = 0, cap = 0;
+  bool pinned = false;  // p came from PinnedStore
   ByteBuf() = default;
   ByteBuf(const ByteBuf& o) {
     reserve(o.n);
@@ -441,7 +506,11 @@ struct ByteBuf {
     n = o.n;
   }
   ByteBuf& operator=(const ByteBuf&) = delete;
-  ~ByteBuf() { std::free(p); }
+  ~ByteBuf() { drop(p, cap, pinned); }
+  static void drop(uint8_t* q, size_t c, bool pin) {
+    if (pin) PinnedStore::release(q, c);
+    else std::free(q);
+  }
   void reserve(size_t want) {
     if (want <= cap) return;
     size_t c = std::max(want, cap * 2);
@@ -452,14 +521,21 @@ struct ByteBuf {
       cap = std::max<size_t>(c, 64);
       return;
     }
-    c = (c + kHuge - 1) & ~(kHuge - 1);
-    void* q = nullptr;
-    if (posix_memalign(&q, kHuge, c) != 0) throw std::bad_alloc();
-    (void)madvise(q, c, MADV_HUGEPAGE);
+    size_t got = 0;
+    uint8_t* q = PinnedStore::acquire(c, &got);
+    bool pin = q != nullptr;
+    if (!q) {
+      got = (c + kHuge - 1) & ~(kHuge - 1);
+      void* v = nullptr;
+      if (posix_memalign(&v, kHuge, got) != 0) throw std::bad_alloc();
+      (void)madvise(v, got, MADV_HUGEPAGE);
+      q = (uint8_t*)v;
+    }
     if (n) par_memcpy(q, p, n);
-    std::free(p);
-    p = (uint8_t*)q;
-    cap = c;
+    drop(p, cap, pinned);
+    p = q;
+    cap = got;
+    pinned = pin;
   }
   // Appends k uninitialised bytes and returns where they start.
   uint8_t* grow(size_t k) {
@@ -520,6 +596,16 @@ void ts_write_field_elements(zp_transcript* t, const uint64_t mod[4], const Fp* 
 void ts_put(zp_transcript* t, const void* p, size_t n) { t->put(p, n); }
 
 void ts_reserve(zp_transcript* t, size_t more) { t->stream.reserve(t->stream.n + more); }
+
+// `more` bytes appended to the stream, uninitialised, for a device copy to
+// fill: their address if the storage is pinned (else null and nothing is
+// appended).  ts_unput(t, more) takes them back after a failed copy.
+uint8_t* ts_grow_pinned(zp_transcript* t, size_t more) {
+  t->stream.reserve(t->stream.n + more);
+  if (!t->stream.pinned) return nullptr;
+  return t->stream.grow(more);
+}
+void ts_unput(zp_transcript* t, size_t n) { t->stream.n -= std::min(n, t->stream.n); }
 
 }  // namespace zpi
 
