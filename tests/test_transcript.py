@@ -119,3 +119,49 @@ def test_absorb_entries_is_absorb_slice_per_entry(zgh, oracle):
     for i in range(24):
         o.absorb_slice(data[40 * i:40 * (i + 1)])
     assert t.get_challenge(32) == o.challenge(32)
+
+
+_SCALAR_TRACE = r'''
+import sys
+import numpy as np
+sys.path[:0] = [sys.argv[1], sys.argv[2]]
+import zipgpu
+rng = np.random.default_rng(7)
+t = zipgpu.PcsProverTranscript()
+out = []
+for n in [0, 1, 63, 64, 65, 1023, 1024, 1025, 8193, 16 * 1024 + 1, 40000, 300000]:
+    t.absorb_slice(rng.integers(0, 256, n, dtype=np.uint8).tobytes())
+    out.append(t.get_challenge(16).hex())
+    out.append(t.get_challenge(100).hex())
+    out.append(str(t.squeeze_challenge_idx(1000003)))
+print(" ".join(out))
+'''
+
+
+def test_scalar_and_simd_hashers_agree(zgh, oracle):
+    # ZIPGPU_HOST_SIMD=0 keeps the hasher on its scalar compression and
+    # one-chunk-at-a-time hashing (a host without AVX2); this process uses
+    # the SIMD paths the host has.  Both must give the oracle's challenges.
+    import os
+    import subprocess
+    import sys
+    import numpy as np
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    runs = []
+    for simd in ("0", "1"):
+        env = dict(os.environ, ZIPGPU_HOST_SIMD=simd)
+        r = subprocess.run([sys.executable, "-c", _SCALAR_TRACE, root,
+                            os.path.join(root, "zinc-plus_amd")],
+                           env=env, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        runs.append(r.stdout.split())
+    rng = np.random.default_rng(7)
+    o = oracle.Transcript()
+    want = []
+    for n in [0, 1, 63, 64, 65, 1023, 1024, 1025, 8193, 16 * 1024 + 1, 40000, 300000]:
+        o.absorb_slice(rng.integers(0, 256, n, dtype=np.uint8).tobytes())
+        want.append(o.challenge(16).hex())
+        want.append(o.challenge(100).hex())
+        want.append(str(o.squeeze_idx(1000003)))
+    assert runs[0] == want
+    assert runs[1] == want

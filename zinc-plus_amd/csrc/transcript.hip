@@ -33,13 +33,13 @@ namespace {
 
 const uint32_t kIV[8] = {0x6A09E667u, 0xBB67AE85u, 0x3C6EF372u, 0xA54FF53Au,
                          0x510E527Fu, 0x9B05688Cu, 0x1F83D9ABu, 0x5BE0CD19u};
-const uint8_t kSchedule[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
+constexpr uint8_t kSchedule[16] = {2, 6, 3, 10, 7, 0, 4, 13, 1, 11, 12, 5, 9, 14, 15, 8};
 enum : uint32_t { kChunkStart = 1, kChunkEnd = 2, kParent = 4, kRoot = 8 };
 
 inline uint32_t rotr(uint32_t x, int r) { return (x >> r) | (x << (32 - r)); }
 
-void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, uint32_t len,
-              uint32_t flags, uint32_t out[16]) {
+void compress_scalar(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter,
+                     uint32_t len, uint32_t flags, uint32_t out[16]) {
   uint32_t v[16] = {cv[0], cv[1], cv[2], cv[3], cv[4], cv[5], cv[6], cv[7],
                     kIV[0], kIV[1], kIV[2], kIV[3], (uint32_t)counter, (uint32_t)(counter >> 32),
                     len, flags};
@@ -72,6 +72,66 @@ void compress(const uint32_t cv[8], const uint32_t m_in[16], uint64_t counter, u
     out[i] = v[i] ^ v[i + 8];
     out[i + 8] = v[i + 8] ^ cv[i];
   }
+}
+
+// The message word order of each of the seven rounds: round r reads
+// m[kMsgOrder.idx[r][i]] where the scalar compression reads its permuted m[i].
+struct MsgOrder {
+  uint8_t idx[7][16];
+};
+constexpr MsgOrder msg_order() {
+  MsgOrder o{};
+  for (int i = 0; i < 16; i++) o.idx[0][i] = (uint8_t)i;
+  for (int r = 1; r < 7; r++)
+    for (int i = 0; i < 16; i++) o.idx[r][i] = o.idx[r - 1][kSchedule[i]];
+  return o;
+}
+constexpr MsgOrder kMsgOrder = msg_order();
+
+// The same compression with the state as four rows of four words in 128-bit
+// registers: the column step runs G on whole rows; the diagonal step first
+// rotates rows 1, 2 and 3 left by one, two and three lanes, so that lane j
+// holds the j-th diagonal, and rotates them back afterwards.  The 32-bit
+// rotates are AVX-512VL's vprord.
+__attribute__((target("avx512f,avx512vl"))) void compress_rows(const uint32_t cv[8],
+                                                              const uint32_t m[16],
+                                                              uint64_t counter, uint32_t len,
+                                                              uint32_t flags, uint32_t out[16]) {
+  const __m128i c0 = _mm_loadu_si128((const __m128i*)cv);
+  const __m128i c1 = _mm_loadu_si128((const __m128i*)(cv + 4));
+  __m128i r0 = c0, r1 = c1;
+  __m128i r2 = _mm_loadu_si128((const __m128i*)kIV);
+  __m128i r3 = _mm_setr_epi32((int)(uint32_t)counter, (int)(uint32_t)(counter >> 32), (int)len,
+                              (int)flags);
+#define ZP_GROW(x, y)                                  \
+  r0 = _mm_add_epi32(_mm_add_epi32(r0, r1), x);        \
+  r3 = _mm_ror_epi32(_mm_xor_si128(r3, r0), 16);       \
+  r2 = _mm_add_epi32(r2, r3);                          \
+  r1 = _mm_ror_epi32(_mm_xor_si128(r1, r2), 12);       \
+  r0 = _mm_add_epi32(_mm_add_epi32(r0, r1), y);        \
+  r3 = _mm_ror_epi32(_mm_xor_si128(r3, r0), 8);        \
+  r2 = _mm_add_epi32(r2, r3);                          \
+  r1 = _mm_ror_epi32(_mm_xor_si128(r1, r2), 7);
+  for (int r = 0; r < 7; r++) {
+    const uint8_t* o = kMsgOrder.idx[r];
+    const __m128i x0 = _mm_setr_epi32((int)m[o[0]], (int)m[o[2]], (int)m[o[4]], (int)m[o[6]]);
+    const __m128i y0 = _mm_setr_epi32((int)m[o[1]], (int)m[o[3]], (int)m[o[5]], (int)m[o[7]]);
+    const __m128i x1 = _mm_setr_epi32((int)m[o[8]], (int)m[o[10]], (int)m[o[12]], (int)m[o[14]]);
+    const __m128i y1 = _mm_setr_epi32((int)m[o[9]], (int)m[o[11]], (int)m[o[13]], (int)m[o[15]]);
+    ZP_GROW(x0, y0)
+    r1 = _mm_shuffle_epi32(r1, _MM_SHUFFLE(0, 3, 2, 1));
+    r2 = _mm_shuffle_epi32(r2, _MM_SHUFFLE(1, 0, 3, 2));
+    r3 = _mm_shuffle_epi32(r3, _MM_SHUFFLE(2, 1, 0, 3));
+    ZP_GROW(x1, y1)
+    r1 = _mm_shuffle_epi32(r1, _MM_SHUFFLE(2, 1, 0, 3));
+    r2 = _mm_shuffle_epi32(r2, _MM_SHUFFLE(1, 0, 3, 2));
+    r3 = _mm_shuffle_epi32(r3, _MM_SHUFFLE(0, 3, 2, 1));
+  }
+#undef ZP_GROW
+  _mm_storeu_si128((__m128i*)out, _mm_xor_si128(r0, r2));
+  _mm_storeu_si128((__m128i*)(out + 4), _mm_xor_si128(r1, r3));
+  _mm_storeu_si128((__m128i*)(out + 8), _mm_xor_si128(r2, c0));
+  _mm_storeu_si128((__m128i*)(out + 12), _mm_xor_si128(r3, c1));
 }
 
 // Eight full chunks at once, one per 32-bit lane of AVX2 vectors: chunk i
@@ -197,14 +257,35 @@ __attribute__((target("avx512f"))) void hash16_chunks_avx512(const uint8_t* p, u
 #undef ZP_G16
 }
 
+// ZIPGPU_HOST_SIMD=0 in the environment keeps the hasher on its scalar code
+// (what a host without AVX2 runs); the digests are the same either way.
+bool host_simd() {
+  static const bool yes = [] {
+    const char* e = std::getenv("ZIPGPU_HOST_SIMD");
+    return !(e && e[0] == '0');
+  }();
+  return yes;
+}
+
 bool host_has_avx2() {
-  static const bool yes = __builtin_cpu_supports("avx2");
+  static const bool yes = host_simd() && __builtin_cpu_supports("avx2");
   return yes;
 }
 
 bool host_has_avx512() {
-  static const bool yes = __builtin_cpu_supports("avx512f");
+  static const bool yes = host_simd() && __builtin_cpu_supports("avx512f");
   return yes;
+}
+
+bool host_has_avx512vl() {
+  static const bool yes = host_has_avx512() && __builtin_cpu_supports("avx512vl");
+  return yes;
+}
+
+void compress(const uint32_t cv[8], const uint32_t m[16], uint64_t counter, uint32_t len,
+              uint32_t flags, uint32_t out[16]) {
+  if (host_has_avx512vl()) compress_rows(cv, m, counter, len, flags, out);
+  else compress_scalar(cv, m, counter, len, flags, out);
 }
 
 void words_le(const uint8_t* b, uint32_t* w, int n) {
@@ -408,8 +489,12 @@ namespace zpi {
 // spreads the first-touch page faults of a freshly grown destination.
 void par_memcpy(void* dst, const void* src, size_t n) {
   const size_t kMin = 8u << 20;
-  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  if (n < kMin || hw == 1) {
+  if (n < kMin) {
+    std::memcpy(dst, src, n);
+    return;
+  }
+  static const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (hw == 1) {
     std::memcpy(dst, src, n);
     return;
   }
@@ -556,6 +641,14 @@ struct zp_transcript {
   void absorb_tag(uint8_t t) { fs.update(&t, 1); }
   void challenge(uint8_t* out, size_t n) {
     fs.xof(out, n);
+    if (n <= 62) {  // 0x12, the bytes, 0x34 in one update
+      uint8_t rec[64];
+      rec[0] = 0x12;
+      std::memcpy(rec + 1, out, n);
+      rec[n + 1] = 0x34;
+      absorb(rec, n + 2);
+      return;
+    }
     absorb_tag(0x12);
     absorb(out, n);
     absorb_tag(0x34);
@@ -580,16 +673,28 @@ uint64_t ts_squeeze_idx(zp_transcript* t, uint64_t cap) {
 // write_field_elements (pcs_transcript.rs:99-142): Uint<4> has no length
 // prefix; each element is absorbed (absorb_random_field) and written as
 // modulus bytes, then inner bytes.
+// The absorbed records (68 bytes each) and the stream bytes (64 each) are
+// laid out in host buffers first and handed over in one call each: BLAKE3
+// is a streaming hash, so the digest does not depend on how the input is
+// split.
 void ts_write_field_elements(zp_transcript* t, const uint64_t mod[4], const Fp* v, size_t n) {
-  for (size_t i = 0; i < n; i++) {
-    t->absorb_tag(0x3);
-    t->absorb(mod, 32);
-    t->absorb_tag(0x5);
-    t->absorb_tag(0x1);
-    t->absorb(v[i].w, 32);
-    t->absorb_tag(0x3);
-    t->put(mod, 32);
-    t->put(v[i].w, 32);
+  constexpr size_t kRec = 68, kOut = 64, kBatch = 1024;
+  std::vector<uint8_t> rec(kRec * std::min(n, kBatch));
+  uint8_t* out = n ? t->stream.grow(kOut * n) : nullptr;
+  for (size_t i0 = 0; i0 < n; i0 += kBatch) {
+    const size_t m = std::min(kBatch, n - i0);
+    for (size_t i = 0; i < m; i++) {
+      uint8_t* r = rec.data() + kRec * i;
+      r[0] = 0x3;
+      std::memcpy(r + 1, mod, 32);
+      r[33] = 0x5;
+      r[34] = 0x1;
+      std::memcpy(r + 35, v[i0 + i].w, 32);
+      r[67] = 0x3;
+      std::memcpy(out + kOut * (i0 + i), mod, 32);
+      std::memcpy(out + kOut * (i0 + i) + 32, v[i0 + i].w, 32);
+    }
+    t->absorb(rec.data(), kRec * m);
   }
 }
 
