@@ -105,6 +105,24 @@ const char* zp_last_error(void);
  * Returns ZP_ERR_INVALID_PARAM for an unknown name or value. */
 int zp_set_option(const char* name, int64_t value);
 
+/* Encoder timing inside commits.  No counterpart in the reference: for
+ * profiling callers that want the encoder's device time as it runs inside
+ * whole commits rather than in a loop of its own.
+ * zp_encode_timing_start creates 2 * max_commits HIP timing events (no
+ * system-scope fence) up front.  Then one zp_commit in `every` (the 1st,
+ * the every+1-th, ...) among those that encode their rows in one sequence
+ * on their stream takes the next pair, while any are left, and records it
+ * around its encoder launches: the bit-plane pre-pass and the encoder
+ * proper for an IPRS binary-poly code.  A timed commit creates nothing; it
+ * adds two event records to its stream.
+ * zp_encode_timing_stop ends the recording, waits for the recorded events,
+ * returns the summed encoder time in ms and the number of timed commits,
+ * and destroys the events.  Batch plans and the server are never timed.
+ * ZP_ERR_INVALID_PARAM: start while on, stop while off, max_commits outside
+ * 1..2^20, every = 0. */
+int zp_encode_timing_start(uint32_t max_commits, uint32_t every);
+int zp_encode_timing_stop(double* total_ms, uint64_t* launches);
+
 /* ---- LinearCode (zip-plus/src/code.rs:10-72) -------------------------- */
 
 /* IprsCode::new(row_len, depth) (zip-plus/src/code/iprs.rs:30-37; errors as

@@ -42,6 +42,17 @@ def test_status_strings_without_device():
     assert lib.zp_status_string(1) == b"invalid parameter"
 
 
+def test_encode_timing_argument_errors():
+    # Checked before any device call: a pool size outside 1..2^20, a zero
+    # sampling interval, and stop without start.
+    from zipgpu import _lib
+    lib = _lib.load()
+    assert lib.zp_encode_timing_start(0, 1) == _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_encode_timing_start((1 << 20) + 1, 1) == _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_encode_timing_start(4, 0) == _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_encode_timing_stop(None, None) == _lib.ZP_ERR_INVALID_PARAM
+
+
 def test_setup_validation():
     # pcs/structs.rs:74-88 asserts; checked on the host before any device work.
     import zipgpu

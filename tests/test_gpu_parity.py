@@ -349,6 +349,34 @@ def test_commit_overlap_matches_sequential(zg, dev, oracle, kind, n, row_len, D)
     assert on.root == off.root == ref["root"]
 
 
+@pytest.mark.parametrize("kind,n,row_len,D,batch", [
+    ("iprs", 1 << 16, 1 << 12, 32, 1), ("iprs", 1 << 14, 1 << 10, 64, 2),
+    ("raa", 1 << 16, 256, 32, 1)])
+def test_encode_timing_inside_commits(zg, dev, oracle, kind, n, row_len, D, batch):
+    # Commits recorded with encoder timing events give the oracle's roots,
+    # one event pair per commit with a positive duration; commits outside
+    # the block are not recorded.
+    xs = [synth.binary_polys(900 + n + i, n, D) for i in range(batch)]
+    code = (zg.IprsCode.new_with_optimal_depth(row_len) if kind == "iprs"
+            else zg.RaaCode(row_len))
+    zp = zg.ZipPlus(zg.BinPolyTypes(D))
+    pp = zp.setup(n, code)
+    xd = [to_dev(x, dev) for x in xs]
+    _, before = zp.commit(pp, xd)
+    with zg.EncodeTiming(3) as et:
+        roots = [zp.commit(pp, xd)[1].root for _ in range(4)]  # the 4th is past the pool
+    with zg.EncodeTiming(8, every=3) as es:  # commits 1, 4 and 7 of 7
+        roots += [zp.commit(pp, xd)[1].root for _ in range(7)]
+    _, after = zp.commit(pp, xd)
+    assert et.launches == 3 and et.total_ms > 0
+    assert es.launches == 3 and es.total_ms > 0
+    if batch == 1:
+        ref = oracle.commit(kind, xs[0].view(np.uint64), pp.num_rows, row_len, D=D)["root"]
+    else:
+        ref = before.root
+    assert roots == [ref] * 11 and before.root == after.root == ref
+
+
 def test_hint_openings_verify(zg, dev, oracle):
     n, row_len, D = 1 << 12, 256, 32
     x = synth.binary_polys(77, 2 * n, D)

@@ -2,6 +2,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -195,13 +196,56 @@ int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void*
 
 namespace {
 
+// Encoder timing inside zp_commit: while it is on, each commit takes the
+// next pair of a pool of timing events (made by zp_encode_timing_start, so a
+// timed commit creates nothing) and records it around its encoder launches
+// on the commit stream.  The events carry no system-scope fence: recording
+// them writes back and invalidates no cache.  zp_encode_timing_stop reads
+// and destroys the pool.  The store is never destroyed, so a commit running
+// during process exit still finds it.
+struct EncodeTiming {
+  std::mutex mu;
+  bool on = false;
+  std::vector<hipEvent_t> pool;  // pairs: pool[2i], pool[2i + 1]
+  size_t used = 0;               // pairs handed out
+  uint32_t every = 1;            // one commit in `every` is timed
+  uint64_t seen = 0;             // commits that asked while on
+  std::vector<uint8_t> ok;       // per pair handed out
+};
+EncodeTiming& encode_timing() {
+  static auto* t = new EncodeTiming;
+  return *t;
+}
+
+// The next pair of the pool (its index in *slot), or false when timing is
+// off or the pool is used up.
+bool timing_begin(hipEvent_t (&e)[2], size_t* slot) {
+  e[0] = e[1] = nullptr;
+  EncodeTiming& t = encode_timing();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (!t.on || 2 * (t.used + 1) > t.pool.size()) return false;
+  if (t.seen++ % t.every != 0) return false;
+  *slot = t.used++;
+  e[0] = t.pool[2 * *slot];
+  e[1] = t.pool[2 * *slot + 1];
+  t.ok.push_back(0);
+  return true;
+}
+
+// Marks the pair of a successful encode as one to read.
+void timing_end(size_t slot, bool ok) {
+  EncodeTiming& t = encode_timing();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (ok && slot < t.ok.size()) t.ok[slot] = 1;
+}
+
 // Encode all rows of all polys, then hash the leaves.  For a single poly
 // whose rows split into whole 1 KiB BLAKE3 chunks, the rows are encoded in
 // groups on `s` while the chunks of finished groups are hashed on the
 // auxiliary stream; the streams join before the chunk trees are merged.
 int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
                            uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
-                           bool allow_overlap) {
+                           bool allow_overlap, bool timed) {
   const uint64_t N = h->N, elem = h->elem, rows_all = num_rows * batch;
   const uint64_t in_stride = num_rows * code->row_len * eval_bytes_of(eval_type, D);
   const uint64_t rpc = (elem <= 1024 && 1024 % elem == 0) ? 1024 / elem : 0;  // rows per chunk
@@ -211,9 +255,19 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
                          batch == 1 && rpc && num_rows % (rpc * kGroups) == 0 &&
                          nchunks >= kGroups;
   if (!pipelined) {
-    for (uint32_t b = 0; b < batch; b++)
-      ZP_TRY(encode_rows_variant(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride,
-                                 num_rows, h->d_cw + b * h->per_poly, s));
+    hipEvent_t te[2] = {nullptr, nullptr};
+    size_t slot = 0;
+    const bool tmd = timed && timing_begin(te, &slot);
+    int rc = ZP_OK;
+    if (tmd && hipEventRecord(te[0], s) != hipSuccess)
+      rc = fail(ZP_ERR_DEVICE, "event record failed");
+    for (uint32_t b = 0; rc == ZP_OK && b < batch; b++)
+      rc = encode_rows_variant(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride,
+                               num_rows, h->d_cw + b * h->per_poly, s);
+    if (tmd && rc == ZP_OK && hipEventRecord(te[1], s) != hipSuccess)
+      rc = fail(ZP_ERR_DEVICE, "event record failed");
+    if (tmd) timing_end(slot, rc == ZP_OK);
+    ZP_TRY(rc);
     return launch_leaf_hash(h->d_cw, rows_all, N, (uint32_t)elem, h->d_layers, s,
                             code->tune_ctree_m);
   }
@@ -282,9 +336,9 @@ int hint_alloc(const zp_code* code, int eval_type, uint32_t D, uint64_t num_rows
 
 int commit_enqueue(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
                    uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
-                   bool allow_overlap) {
+                   bool allow_overlap, bool timed) {
   ZP_TRY(commit_encode_and_hash(code, eval_type, D, d_evals, num_rows, batch, h, s,
-                                allow_overlap));
+                                allow_overlap, timed));
   return launch_merkle_layers(h->d_layers, h->N, s);
 }
 
@@ -335,6 +389,66 @@ int zp_set_option(const char* name, int64_t value) {
     return ZP_OK;
   }
   return fail(ZP_ERR_INVALID_PARAM, std::string("unknown option ") + name);
+}
+
+int zp_encode_timing_start(uint32_t max_commits, uint32_t every) {
+  auto& t = zpi::encode_timing();
+  std::lock_guard<std::mutex> g(t.mu);
+  if (t.on) return fail(ZP_ERR_INVALID_PARAM, "encoder timing is already on");
+  if (max_commits == 0 || max_commits > (1u << 20))
+    return fail(ZP_ERR_INVALID_PARAM, "max_commits must be within 1..2^20");
+  if (every == 0) return fail(ZP_ERR_INVALID_PARAM, "every must be positive");
+  std::vector<hipEvent_t> pool(2 * (size_t)max_commits, nullptr);
+  for (auto& e : pool)
+    if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
+      (void)hipGetLastError();
+      for (auto& x : pool)
+        if (x) (void)hipEventDestroy(x);
+      return fail(ZP_ERR_DEVICE, "timing event creation failed");
+    }
+  t.pool.swap(pool);
+  t.used = 0;
+  t.every = every;
+  t.seen = 0;
+  t.ok.clear();
+  t.on = true;
+  return ZP_OK;
+}
+
+int zp_encode_timing_stop(double* total_ms, uint64_t* launches) {
+  auto& t = zpi::encode_timing();
+  std::vector<hipEvent_t> pool;
+  std::vector<uint8_t> ok;
+  size_t used = 0;
+  {
+    std::lock_guard<std::mutex> g(t.mu);
+    if (!t.on) return fail(ZP_ERR_INVALID_PARAM, "encoder timing is not on");
+    t.on = false;
+    pool.swap(t.pool);
+    ok.swap(t.ok);
+    used = t.used;
+    t.used = 0;
+  }
+  double sum = 0.0;
+  uint64_t n = 0;
+  int rc = ZP_OK;
+  for (size_t i = 0; i < used; i++) {
+    // Every handed-out pair is waited for: a failed encode may have
+    // recorded its opening event.
+    if (hipEventSynchronize(pool[2 * i]) != hipSuccess)
+      rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "encoder timing events failed") : rc;
+    if (!ok[i]) continue;
+    float ms = 0.0f;
+    if (hipEventSynchronize(pool[2 * i + 1]) != hipSuccess ||
+        hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]) != hipSuccess)
+      rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "encoder timing events failed") : rc;
+    sum += ms;
+    n++;
+  }
+  for (auto e : pool) (void)hipEventDestroy(e);
+  if (total_ms) *total_ms = rc == ZP_OK ? sum : 0.0;
+  if (launches) *launches = rc == ZP_OK ? n : 0;
+  return rc;
 }
 
 int zp_code_create(int kind, uint32_t rep, uint64_t row_len, int32_t depth, zp_code** out) {
@@ -512,7 +626,8 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_eval
   ZP_TRY(hint_alloc(code, eval_type, D, num_rows, batch, s, &h));
   uint8_t* pin = pinned_scratch(32);
   int rc = pin ? ZP_OK : fail(ZP_ERR_OOM, "pinned staging allocation failed");
-  if (rc == ZP_OK) rc = commit_enqueue(code, eval_type, D, d_evals, num_rows, batch, h, s, true);
+  if (rc == ZP_OK)
+    rc = commit_enqueue(code, eval_type, D, d_evals, num_rows, batch, h, s, true, true);
   if (rc == ZP_OK && hipMemcpyAsync(pin, h->d_layers + (2 * h->N - 2) * 32, 32,
                                     hipMemcpyDeviceToHost, s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "root copy failed");

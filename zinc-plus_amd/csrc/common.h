@@ -166,9 +166,11 @@ bool raa_full_row_eligible(const zp_code* c, uint32_t D);
 // stream without synchronising (the root stays in the hint's last layer).
 int hint_alloc(const zp_code* code, int eval_type, uint32_t D, uint64_t num_rows, uint32_t batch,
                hipStream_t s, zp_hint** out);
+// `timed` (zp_commit only): the encoder launches are bracketed by timing
+// events when zp_encode_timing_start is in effect.
 int commit_enqueue(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
                    uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
-                   bool allow_overlap);
+                   bool allow_overlap, bool timed = false);
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
                         uint64_t num_rows, void* out, hipStream_t s);
 // The variant encode_rows_variant runs for an IPRS binary-poly code: true

@@ -96,6 +96,33 @@ def set_option(name, value):
     _check(L.load().zp_set_option(name.encode(), int(value)))
 
 
+class EncodeTiming:
+    """zp_encode_timing_start/stop as a context: the encoder's device time
+    inside one zp_commit in `every` run in the block (the first, the
+    every+1-th, ...), up to `max_commits` of them, by HIP events on the
+    commit stream (created before the block).  After the block, `total_ms`
+    and `launches` hold the sum and the number of timed commits, and
+    `mean_ms` their ratio."""
+
+    def __init__(self, max_commits=1024, every=1):
+        self.max_commits, self.every = int(max_commits), int(every)
+        self.total_ms, self.launches = 0.0, 0
+
+    def __enter__(self):
+        _check(L.load().zp_encode_timing_start(self.max_commits, self.every))
+        return self
+
+    def __exit__(self, *exc):
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        _check(L.load().zp_encode_timing_stop(ctypes.byref(ms), ctypes.byref(n)))
+        self.total_ms, self.launches = ms.value, n.value
+        return False
+
+    @property
+    def mean_ms(self):
+        return self.total_ms / self.launches if self.launches else float("nan")
+
+
 def _check(rc):
     if rc != L.ZP_OK:
         raise _ERRORS.get(rc, ZipError)(rc, L.last_error())
