@@ -108,20 +108,27 @@ int zp_set_option(const char* name, int64_t value);
 /* Encoder timing inside commits.  No counterpart in the reference: for
  * profiling callers that want the encoder's device time as it runs inside
  * whole commits rather than in a loop of its own.
- * zp_encode_timing_start creates 2 * max_commits HIP timing events (no
+ * zp_encode_timing_start creates 3 * max_commits HIP timing events (no
  * system-scope fence) up front.  Then one zp_commit in `every` (the 1st,
  * the every+1-th, ...) among those that encode their rows in one sequence
- * on their stream takes the next pair, while any are left, and records it
- * around its encoder launches: the bit-plane pre-pass and the encoder
- * proper for an IPRS binary-poly code.  A timed commit creates nothing; it
- * adds two event records to its stream.
+ * on their stream takes the next three, while any are left, and records
+ * them before its encoder, before the encoder's last kernel and after it.
+ * For an IPRS binary-poly code on the matrix cores the last kernel is the
+ * encoder proper, after the bit-plane pre-pass; other encoders record the
+ * middle event at the start.  A timed commit creates nothing; it adds three
+ * event records to its stream.
  * zp_encode_timing_stop ends the recording, waits for the recorded events,
- * returns the summed encoder time in ms and the number of timed commits,
- * and destroys the events.  Batch plans and the server are never timed.
+ * and destroys them.  It returns the number of timed commits and two sums
+ * in ms: of the whole encoder spans, and of the last-kernel spans.  The
+ * last-kernel span is the tighter measure of a kernel: the stream is busy
+ * with the pre-pass while the host submits the last kernel.  The first event
+ * of a whole span is stamped as soon as the idle stream reaches it, so the
+ * span also holds the host's submission of the first kernel.
+ * Batch plans and the server are never timed.
  * ZP_ERR_INVALID_PARAM: start while on, stop while off, max_commits outside
  * 1..2^20, every = 0. */
 int zp_encode_timing_start(uint32_t max_commits, uint32_t every);
-int zp_encode_timing_stop(double* total_ms, uint64_t* launches);
+int zp_encode_timing_stop(double* total_ms, double* last_ms, uint64_t* launches);
 
 /* ---- LinearCode (zip-plus/src/code.rs:10-72) -------------------------- */
 

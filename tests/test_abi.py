@@ -50,7 +50,7 @@ def test_encode_timing_argument_errors():
     assert lib.zp_encode_timing_start(0, 1) == _lib.ZP_ERR_INVALID_PARAM
     assert lib.zp_encode_timing_start((1 << 20) + 1, 1) == _lib.ZP_ERR_INVALID_PARAM
     assert lib.zp_encode_timing_start(4, 0) == _lib.ZP_ERR_INVALID_PARAM
-    assert lib.zp_encode_timing_stop(None, None) == _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_encode_timing_stop(None, None, None) == _lib.ZP_ERR_INVALID_PARAM
 
 
 def test_setup_validation():

@@ -166,7 +166,14 @@ bool iprs_uses_mfma(const zp_code* c, uint32_t D) {
 
 // Encodes with the variant in force for this code (types already checked).
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
-                        uint64_t num_rows, void* out, hipStream_t s) {
+                        uint64_t num_rows, void* out, hipStream_t s, hipEvent_t mid) {
+  const bool mfma_binary = !c->checked && c->kind == ZP_CODE_IPRS &&
+                           eval_type == ZP_EVAL_BINPOLY && iprs_uses_mfma(c, D);
+  if (mfma_binary)
+    return launch_iprs_binary_mfma(c, (const uint64_t*)evals, num_rows, D, (int64_t*)out, s,
+                                   pick(g_force_iprs, c->tune_iprs, 0) != 2, mid);
+  if (mid && hipEventRecord(mid, s) != hipSuccess)
+    return fail(ZP_ERR_DEVICE, "event record failed");
   if (c->checked) {
     bool done = false;
     ZP_TRY(encode_checked(c, eval_type, D, evals, num_rows, out, s, &done));
@@ -196,43 +203,43 @@ int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void*
 
 namespace {
 
-// Encoder timing inside zp_commit: while it is on, each commit takes the
-// next pair of a pool of timing events (made by zp_encode_timing_start, so a
-// timed commit creates nothing) and records it around its encoder launches
-// on the commit stream.  The events carry no system-scope fence: recording
+// Encoder timing inside zp_commit: while it is on, a timed commit takes the
+// next three events of a pool of timing events (made by
+// zp_encode_timing_start, so a timed commit creates nothing) and records
+// them on the commit stream before its encoder, before the encoder's last
+// kernel and after it.  The events carry no system-scope fence: recording
 // them writes back and invalidates no cache.  zp_encode_timing_stop reads
 // and destroys the pool.  The store is never destroyed, so a commit running
 // during process exit still finds it.
 struct EncodeTiming {
   std::mutex mu;
   bool on = false;
-  std::vector<hipEvent_t> pool;  // pairs: pool[2i], pool[2i + 1]
-  size_t used = 0;               // pairs handed out
+  std::vector<hipEvent_t> pool;  // triples: pool[3i .. 3i + 2]
+  size_t used = 0;               // triples handed out
   uint32_t every = 1;            // one commit in `every` is timed
   uint64_t seen = 0;             // commits that asked while on
-  std::vector<uint8_t> ok;       // per pair handed out
+  std::vector<uint8_t> ok;       // per triple handed out
 };
 EncodeTiming& encode_timing() {
   static auto* t = new EncodeTiming;
   return *t;
 }
 
-// The next pair of the pool (its index in *slot), or false when timing is
-// off or the pool is used up.
-bool timing_begin(hipEvent_t (&e)[2], size_t* slot) {
-  e[0] = e[1] = nullptr;
+// The next triple of the pool (its index in *slot), or false when timing is
+// off, this commit is not sampled or the pool is used up.
+bool timing_begin(hipEvent_t (&e)[3], size_t* slot) {
+  e[0] = e[1] = e[2] = nullptr;
   EncodeTiming& t = encode_timing();
   std::lock_guard<std::mutex> g(t.mu);
-  if (!t.on || 2 * (t.used + 1) > t.pool.size()) return false;
+  if (!t.on || 3 * (t.used + 1) > t.pool.size()) return false;
   if (t.seen++ % t.every != 0) return false;
   *slot = t.used++;
-  e[0] = t.pool[2 * *slot];
-  e[1] = t.pool[2 * *slot + 1];
+  for (int i = 0; i < 3; i++) e[i] = t.pool[3 * *slot + i];
   t.ok.push_back(0);
   return true;
 }
 
-// Marks the pair of a successful encode as one to read.
+// Marks the triple of a successful encode as one to read.
 void timing_end(size_t slot, bool ok) {
   EncodeTiming& t = encode_timing();
   std::lock_guard<std::mutex> g(t.mu);
@@ -255,16 +262,17 @@ int commit_encode_and_hash(const zp_code* code, int eval_type, uint32_t D, const
                          batch == 1 && rpc && num_rows % (rpc * kGroups) == 0 &&
                          nchunks >= kGroups;
   if (!pipelined) {
-    hipEvent_t te[2] = {nullptr, nullptr};
+    hipEvent_t te[3] = {nullptr, nullptr, nullptr};
     size_t slot = 0;
     const bool tmd = timed && timing_begin(te, &slot);
     int rc = ZP_OK;
     if (tmd && hipEventRecord(te[0], s) != hipSuccess)
       rc = fail(ZP_ERR_DEVICE, "event record failed");
-    for (uint32_t b = 0; rc == ZP_OK && b < batch; b++)
+    for (uint32_t b = 0; rc == ZP_OK && b < batch; b++)  // the last poly's last kernel
       rc = encode_rows_variant(code, eval_type, D, (const uint8_t*)d_evals + b * in_stride,
-                               num_rows, h->d_cw + b * h->per_poly, s);
-    if (tmd && rc == ZP_OK && hipEventRecord(te[1], s) != hipSuccess)
+                               num_rows, h->d_cw + b * h->per_poly, s,
+                               b + 1 == batch ? te[1] : nullptr);
+    if (tmd && rc == ZP_OK && hipEventRecord(te[2], s) != hipSuccess)
       rc = fail(ZP_ERR_DEVICE, "event record failed");
     if (tmd) timing_end(slot, rc == ZP_OK);
     ZP_TRY(rc);
@@ -398,7 +406,7 @@ int zp_encode_timing_start(uint32_t max_commits, uint32_t every) {
   if (max_commits == 0 || max_commits > (1u << 20))
     return fail(ZP_ERR_INVALID_PARAM, "max_commits must be within 1..2^20");
   if (every == 0) return fail(ZP_ERR_INVALID_PARAM, "every must be positive");
-  std::vector<hipEvent_t> pool(2 * (size_t)max_commits, nullptr);
+  std::vector<hipEvent_t> pool(3 * (size_t)max_commits, nullptr);
   for (auto& e : pool)
     if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) {
       (void)hipGetLastError();
@@ -415,7 +423,7 @@ int zp_encode_timing_start(uint32_t max_commits, uint32_t every) {
   return ZP_OK;
 }
 
-int zp_encode_timing_stop(double* total_ms, uint64_t* launches) {
+int zp_encode_timing_stop(double* total_ms, double* last_ms, uint64_t* launches) {
   auto& t = zpi::encode_timing();
   std::vector<hipEvent_t> pool;
   std::vector<uint8_t> ok;
@@ -429,24 +437,28 @@ int zp_encode_timing_stop(double* total_ms, uint64_t* launches) {
     used = t.used;
     t.used = 0;
   }
-  double sum = 0.0;
+  double sum = 0.0, sum_last = 0.0;
   uint64_t n = 0;
   int rc = ZP_OK;
   for (size_t i = 0; i < used; i++) {
-    // Every handed-out pair is waited for: a failed encode may have
-    // recorded its opening event.
-    if (hipEventSynchronize(pool[2 * i]) != hipSuccess)
-      rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "encoder timing events failed") : rc;
-    if (!ok[i]) continue;
-    float ms = 0.0f;
-    if (hipEventSynchronize(pool[2 * i + 1]) != hipSuccess ||
-        hipEventElapsedTime(&ms, pool[2 * i], pool[2 * i + 1]) != hipSuccess)
-      rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "encoder timing events failed") : rc;
-    sum += ms;
+    hipEvent_t* e = &pool[3 * i];
+    // Every handed-out triple is waited for: a failed encode may have
+    // recorded some of its events.
+    for (int k = 0; k < 3; k++)
+      if (hipEventSynchronize(e[k]) != hipSuccess)
+        rc = rc == ZP_OK ? fail(ZP_ERR_DEVICE, "encoder timing events failed") : rc;
+    if (!ok[i] || rc != ZP_OK) continue;
+    float all = 0.0f, last = 0.0f;
+    if (hipEventElapsedTime(&all, e[0], e[2]) != hipSuccess ||
+        hipEventElapsedTime(&last, e[1], e[2]) != hipSuccess)
+      rc = fail(ZP_ERR_DEVICE, "encoder timing events failed");
+    sum += all;
+    sum_last += last;
     n++;
   }
   for (auto e : pool) (void)hipEventDestroy(e);
   if (total_ms) *total_ms = rc == ZP_OK ? sum : 0.0;
+  if (last_ms) *last_ms = rc == ZP_OK ? sum_last : 0.0;
   if (launches) *launches = rc == ZP_OK ? n : 0;
   return rc;
 }

@@ -151,8 +151,11 @@ bool iprs_mfma_eligible(const zp_code* c, uint32_t D);
 uint64_t iprs_mfma_scratch_bytes(const zp_code* c, uint64_t rows, uint32_t D);
 // spread: r-block rows bd/16 apart (d_afrag), else 16 consecutive rows
 // (d_afrag_c); iprs_encoder variants 0 and 2.
+// `mid` (or null) is recorded between the bit-plane pre-pass and the
+// encoder proper.
 int launch_iprs_binary_mfma(const zp_code* c, const uint64_t* evals, uint64_t rows, uint32_t D,
-                            int64_t* out, hipStream_t s, bool spread = true);
+                            int64_t* out, hipStream_t s, bool spread = true,
+                            hipEvent_t mid = nullptr);
 int launch_iprs_i32(const zp_code* c, const int32_t* evals, uint64_t rows, void* out,
                     hipStream_t s);
 bool wide_mfma_eligible(const zp_code* c);
@@ -171,8 +174,11 @@ int hint_alloc(const zp_code* code, int eval_type, uint32_t D, uint64_t num_rows
 int commit_enqueue(const zp_code* code, int eval_type, uint32_t D, const void* d_evals,
                    uint64_t num_rows, uint32_t batch, zp_hint* h, hipStream_t s,
                    bool allow_overlap, bool timed = false);
+// `mid` (or null) is recorded before the encoder's last kernel: after the
+// bit-plane pre-pass on the IPRS matrix-core path, before the first launch
+// on the others.
 int encode_rows_variant(const zp_code* c, int eval_type, uint32_t D, const void* evals,
-                        uint64_t num_rows, void* out, hipStream_t s);
+                        uint64_t num_rows, void* out, hipStream_t s, hipEvent_t mid = nullptr);
 // The variant encode_rows_variant runs for an IPRS binary-poly code: true
 // for the matrix-core encoder.
 bool iprs_uses_mfma(const zp_code* c, uint32_t D);

@@ -448,7 +448,7 @@ int launch_main(const zp_code* cd, const uint32_t* planes, uint64_t rows, uint32
 
 template <int NC>
 int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D, int64_t* out,
-              hipStream_t s, bool spread) {
+              hipStream_t s, bool spread, hipEvent_t mid) {
   const uint32_t S = cd->mfma_ksteps, S2 = (S + 1) / 2;
   const uint32_t Dp = (D + 15) / 16 * 16;  // planes above D are computed, not stored
   const uint64_t bytes = rows * NC * 4 * S2 * Dp * sizeof(uint32_t);
@@ -458,6 +458,8 @@ int launch_nc(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t 
   hipLaunchKernelGGL((bitplane_kernel<NC>), dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0,
                      s, evals, planes, (uint32_t)cd->row_len, cd->bl, S, Dp, cd->depth, rows);
   int rc = hipGetLastError() == hipSuccess ? ZP_OK : fail(ZP_ERR_DEVICE, "bitplane launch");
+  if (rc == ZP_OK && mid && hipEventRecord(mid, s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "event record failed");
   if (rc == ZP_OK) {
     switch (S) {
       case 1: rc = launch_main<NC, 1>(cd, planes, rows, D, out, s, spread); break;
@@ -484,10 +486,14 @@ bool iprs_mfma_eligible(const zp_code* cd, uint32_t D) {
 }
 
 int launch_iprs_binary_mfma(const zp_code* cd, const uint64_t* evals, uint64_t rows, uint32_t D,
-                            int64_t* out, hipStream_t s, bool spread) {
-  if (rows == 0) return ZP_OK;
-  if (cd->depth == 1) return launch_nc<8>(cd, evals, rows, D, out, s, spread);
-  return launch_nc<64>(cd, evals, rows, D, out, s, spread);
+                            int64_t* out, hipStream_t s, bool spread, hipEvent_t mid) {
+  if (rows == 0) {
+    if (mid && hipEventRecord(mid, s) != hipSuccess)
+      return fail(ZP_ERR_DEVICE, "event record failed");
+    return ZP_OK;
+  }
+  if (cd->depth == 1) return launch_nc<8>(cd, evals, rows, D, out, s, spread, mid);
+  return launch_nc<64>(cd, evals, rows, D, out, s, spread, mid);
 }
 
 }  // namespace zpi

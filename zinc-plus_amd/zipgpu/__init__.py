@@ -100,27 +100,34 @@ class EncodeTiming:
     """zp_encode_timing_start/stop as a context: the encoder's device time
     inside one zp_commit in `every` run in the block (the first, the
     every+1-th, ...), up to `max_commits` of them, by HIP events on the
-    commit stream (created before the block).  After the block, `total_ms`
-    and `launches` hold the sum and the number of timed commits, and
-    `mean_ms` their ratio."""
+    commit stream (created before the block).  After the block, `launches`
+    is the number of timed commits, `total_ms` the sum of their whole
+    encoder spans and `last_ms` the sum of their last-kernel spans (the
+    matrix-core IPRS encoder after its bit-plane pre-pass); `mean_ms` and
+    `mean_last_ms` are the averages."""
 
     def __init__(self, max_commits=1024, every=1):
         self.max_commits, self.every = int(max_commits), int(every)
-        self.total_ms, self.launches = 0.0, 0
+        self.total_ms, self.last_ms, self.launches = 0.0, 0.0, 0
 
     def __enter__(self):
         _check(L.load().zp_encode_timing_start(self.max_commits, self.every))
         return self
 
     def __exit__(self, *exc):
-        ms, n = ctypes.c_double(), ctypes.c_uint64()
-        _check(L.load().zp_encode_timing_stop(ctypes.byref(ms), ctypes.byref(n)))
-        self.total_ms, self.launches = ms.value, n.value
+        ms, last, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint64()
+        _check(L.load().zp_encode_timing_stop(ctypes.byref(ms), ctypes.byref(last),
+                                              ctypes.byref(n)))
+        self.total_ms, self.last_ms, self.launches = ms.value, last.value, n.value
         return False
 
     @property
     def mean_ms(self):
         return self.total_ms / self.launches if self.launches else float("nan")
+
+    @property
+    def mean_last_ms(self):
+        return self.last_ms / self.launches if self.launches else float("nan")
 
 
 def _check(rc):

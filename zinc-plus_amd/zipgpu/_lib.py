@@ -69,7 +69,8 @@ SIGNATURES = {
     "zp_last_error": (ctypes.c_char_p, []),
     "zp_set_option": (_int, [ctypes.c_char_p, ctypes.c_int64]),
     "zp_encode_timing_start": (_int, [_u32, _u32]),
-    "zp_encode_timing_stop": (_int, [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
+    "zp_encode_timing_stop": (_int, [ctypes.POINTER(ctypes.c_double),
+                                     ctypes.POINTER(ctypes.c_double), ctypes.POINTER(_u64)]),
     "zp_code_create": (_int, [_int, _u32, _u64, _i32, ctypes.POINTER(_c_void_p)]),
     "zp_code_destroy": (None, [_c_void_p]),
     "zp_code_set_checked": (_int, [_c_void_p, _int]),
