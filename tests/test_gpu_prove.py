@@ -6,6 +6,7 @@ import numpy as np
 import pytest
 import torch
 
+from test_opening_oracle import expected_proof_bytes
 from zipgpu import synth
 
 pytestmark = pytest.mark.gpu
@@ -73,6 +74,8 @@ def run_case(zg, dev, oracle, kind, row_len, num_rows, D, batch, L, eval_type, p
     assert ev == want_ev
     assert len(got) == len(ts.stream)
     assert got == bytes(ts.stream)
+    assert len(got) == expected_proof_bytes(num_rows, row_len, 4 * row_len, elem, L, batch,
+                                            num_openings)
     tv = oracle.Transcript(got)
     tv.absorb_slice(comm.root)
     tv.random_field_cfg()

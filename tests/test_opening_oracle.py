@@ -161,6 +161,28 @@ def _setup(oracle, kind, row_len, num_rows, D, batch, eval_type, seed):
     return ev, c, elem, coeff_bytes
 
 
+def expected_proof_bytes(num_rows, row_len, N, elem, L, batch, num_openings):
+    """The proof-stream size of phase_commit.rs:741-771 (the reference's
+    proof_size_is_correct_for_parameters) for MontyField<4>: b is one vector
+    of num_rows field elements summed over the polys (phase_prove.rs:179-193),
+    each 32 bytes of modulus and 32 of value, with no length prefix because
+    Uint<4> is ConstTranscribable (transcript/src/traits.rs:88-89); the
+    combined row is row_len CombR = Int<L>; each opening is the column of
+    every poly's rows plus a Merkle proof of three u64 and one 32-byte hash
+    per level of the tree over N leaves."""
+    depth = (N - 1).bit_length()  # log2 of N.next_power_of_two()
+    return (num_rows * 64 + row_len * 8 * L +
+            num_openings * (batch * num_rows * elem + 3 * 8 + depth * 32))
+
+
+def test_proof_size_formula_matches_the_reference_helper():
+    # At batch 1 the restatement is the reference helper with a zero-length
+    # prefix; the cfg2 proof (2^20 BinaryPoly<32>, IPRS row 2^14 x 64, Int<5>,
+    # 147 openings) is 3146696 bytes.
+    assert expected_proof_bytes(64, 1 << 14, 1 << 16, 256, 5, 1, 147) == 3146696
+    assert expected_proof_bytes(1, 512, 2048, 8, 5, 1, 147) == 64 + 512 * 40 + 147 * (8 + 24 + 11 * 32)
+
+
 def _encoder(oracle, kind, row_len, L):
     if kind == "iprs":
         return lambda comb: oracle.iprs_encode_wide(comb.reshape(1, row_len, L), row_len, -1, L)[0]
@@ -198,6 +220,7 @@ def test_prove_verify_roundtrip_and_eval_closed_form(oracle, kind, row_len, num_
             w = _alpha_proj(x, alphas[b], D) if eval_type == "binpoly" else x
             want += w * eq[i]
     assert e == want % p
+    assert len(ts.stream) == expected_proof_bytes(num_rows, row_len, N, elem, L, batch, 20)
     tv = oracle.Transcript(ts.stream)
     tv.absorb_slice(c["root"])
     assert tv.random_field_cfg() == p
