@@ -42,6 +42,17 @@ def test_status_strings_without_device():
     assert lib.zp_status_string(1) == b"invalid parameter"
 
 
+def test_commit_host_null_arguments():
+    # Rejected before any device call.
+    from zipgpu import _lib
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    root = (ctypes.c_uint8 * 32)()
+    assert lib.zp_commit_host(None, 1, 32, None, 1, 1, None, root, None) == _lib.ZP_ERR_INVALID_PARAM
+    assert lib.zp_commit_host(None, 1, 32, None, 1, 1, ctypes.byref(h), None,
+                              None) == _lib.ZP_ERR_INVALID_PARAM
+
+
 def test_encode_timing_argument_errors():
     # Checked before any device call: a pool size outside 1..2^20, a zero
     # sampling interval, and stop without start.

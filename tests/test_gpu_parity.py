@@ -377,6 +377,48 @@ def test_encode_timing_inside_commits(zg, dev, oracle, kind, n, row_len, D, batc
     assert roots == [ref] * 11 and before.root == after.root == ref
 
 
+@pytest.mark.parametrize("kind,n,row_len,D,batch,eval_type,pinned", [
+    ("iprs", 1 << 20, 1 << 14, 32, 1, "binpoly", True),   # cfg2: 64 rows, four groups of 16
+    ("iprs", 1 << 20, 1 << 14, 32, 1, "binpoly", False),
+    ("iprs", 1 << 13, 1 << 10, 64, 2, "binpoly", False),  # 8 rows per poly, two polys
+    ("raa", 1 << 16, 256, 32, 1, "binpoly", True),
+    ("raa_signflip", 1 << 12, 64, 16, 3, "binpoly", False),
+    ("iprs", 1 << 15, 1 << 14, 1, 1, "i32", True),          # 2 rows: one group
+    ("iprs", 1 << 12, 1 << 12, 1, 1, "i32", False)])        # 1 row
+def test_commit_from_host_memory(zg, dev, oracle, kind, n, row_len, D, batch, eval_type, pinned):
+    # zp_commit_host (row groups copied on a second stream while earlier
+    # groups encode) equals zp_commit of the same bytes on the device, and
+    # the oracle.
+    say("commit_host", kind, n, row_len, D, batch, eval_type, pinned)
+    if eval_type == "binpoly":
+        x = synth.binary_polys(600 + n + D, batch * n, D)
+        types, ox = zg.BinPolyTypes(D), x.view(np.uint64)
+    else:
+        x = synth.i32_evals(600 + n, batch * n)
+        types, ox = zg.IntTypes(), x
+    code = {"iprs": lambda: zg.IprsCode.new_with_optimal_depth(row_len),
+            "raa": lambda: zg.RaaCode(row_len),
+            "raa_signflip": lambda: zg.RaaSignFlippingCode(row_len)}[kind]()
+    zp = zg.ZipPlus(types)
+    pp = zp.setup(n, code)
+    host = [torch.from_numpy(np.ascontiguousarray(x[b * n:(b + 1) * n])) for b in range(batch)]
+    if pinned:
+        host = [t.pin_memory() for t in host]
+    assert not host[0].is_cuda
+    hh, hc = zp.commit(pp, host)
+    dh, dc = zp.commit(pp, [t.to(dev) for t in host])
+    ref = oracle.commit(kind, ox, pp.num_rows, row_len, batch=batch, D=D, eval_type=eval_type)
+    assert hc.root == dc.root == ref["root"] and hc.batch_size == batch
+    for b in range(batch):
+        assert hh.codewords(b).tobytes() == dh.codewords(b).tobytes()
+    # The host buffers may be reused at once: a second commit of changed
+    # inputs gives the changed root.
+    host[0][:8] ^= 1
+    _, hc2 = zp.commit(pp, host)
+    _, dc2 = zp.commit(pp, [t.to(dev) for t in host])
+    assert hc2.root == dc2.root != hc.root
+
+
 def test_hint_openings_verify(zg, dev, oracle):
     n, row_len, D = 1 << 12, 256, 32
     x = synth.binary_polys(77, 2 * n, D)

@@ -1,5 +1,7 @@
 // api.hip -- the C ABI declared in include/zipgpu.h.
 #include <algorithm>
+#include <array>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <mutex>
@@ -646,6 +648,99 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t D, const void* d_eval
   if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
     rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
   if (rc == ZP_OK) std::memcpy(h_root, pin, 32);
+  if (rc != ZP_OK) {
+    zp_hint_free(h);
+    return rc;
+  }
+  *hint_out = h;
+  return ZP_OK;
+}
+
+namespace zpi {
+namespace {
+
+// The five ordering events of zp_commit_host, made once per host thread and
+// device and reused by every later call (creating and destroying them took
+// host time ahead of the first copy).
+int host_commit_events(hipEvent_t (&ev)[5]) {
+  static thread_local std::map<int, std::array<hipEvent_t, 5>> cache;
+  int dev = 0;
+  ZP_HIP(hipGetDevice(&dev));
+  auto it = cache.find(dev);
+  if (it == cache.end()) {
+    std::array<hipEvent_t, 5> e{};
+    for (auto& x : e)
+      if (hipEventCreateWithFlags(&x, hipEventDisableTiming) != hipSuccess) {
+        for (auto y : e)
+          if (y) (void)hipEventDestroy(y);
+        return fail(ZP_ERR_DEVICE, "event creation failed");
+      }
+    it = cache.emplace(dev, e).first;
+  }
+  for (int i = 0; i < 5; i++) ev[i] = it->second[i];
+  return ZP_OK;
+}
+
+}  // namespace
+}  // namespace zpi
+
+int zp_commit_host(const zp_code* code, int eval_type, uint32_t D, const void* h_evals,
+                   uint64_t num_rows, uint32_t batch, zp_hint** hint_out, uint8_t h_root[32],
+                   zp_stream stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!code || !hint_out || !h_evals || !h_root)
+    return fail(ZP_ERR_INVALID_PARAM, "null argument");
+  *hint_out = nullptr;
+  zp_hint* h = nullptr;
+  ZP_TRY(hint_alloc(code, eval_type, D, num_rows, batch, s, &h));
+  const uint64_t row_bytes = code->row_len * eval_bytes_of(eval_type, D);
+  const uint64_t per_poly_in = num_rows * row_bytes, elem = h->elem, N = h->N;
+  // Rows in up to four groups per poly: group g's rows are copied on the
+  // copy stream while the earlier groups encode on `s`.
+  const uint64_t groups = num_rows >= 4 ? 4 : 1;
+  const uint64_t rows_g = (num_rows + groups - 1) / groups;
+  uint8_t* d_in = nullptr;
+  hipStream_t cs = nullptr;
+  hipEvent_t ev[5] = {};  // ev[0]: the input buffer is ready for the copy stream
+  int rc = dev_alloc((void**)&d_in, batch * per_poly_in, s);
+  if (rc == ZP_OK) rc = aux_stream(&cs);
+  if (rc == ZP_OK) rc = zpi::host_commit_events(ev);
+  if (rc == ZP_OK &&
+      (hipEventRecord(ev[0], s) != hipSuccess || hipStreamWaitEvent(cs, ev[0], 0) != hipSuccess))
+    rc = fail(ZP_ERR_DEVICE, "stream ordering failed");
+  // Each event is recorded again only after `s` has enqueued its wait on
+  // the previous record, so the wait sees the record it follows.
+  for (uint32_t b = 0; rc == ZP_OK && b < batch; b++)
+    for (uint64_t g = 0; rc == ZP_OK && g < groups; g++) {
+      const uint64_t r0 = g * rows_g, nr = std::min(rows_g, num_rows - std::min(r0, num_rows));
+      if (nr == 0) break;
+      const uint64_t off = b * per_poly_in + r0 * row_bytes;
+      hipEvent_t e = ev[1 + g];
+      if (hipMemcpyAsync(d_in + off, (const uint8_t*)h_evals + off, nr * row_bytes,
+                         hipMemcpyHostToDevice, cs) != hipSuccess ||
+          hipEventRecord(e, cs) != hipSuccess || hipStreamWaitEvent(s, e, 0) != hipSuccess)
+        rc = fail(ZP_ERR_DEVICE, "input copy failed");
+      if (rc == ZP_OK)
+        rc = encode_rows_variant(code, eval_type, D, d_in + off, nr,
+                                 h->d_cw + b * h->per_poly + r0 * N * elem, s);
+    }
+  if (rc == ZP_OK)
+    rc = launch_leaf_hash(h->d_cw, num_rows * batch, N, (uint32_t)elem, h->d_layers, s,
+                          code->tune_ctree_m);
+  if (rc == ZP_OK) rc = launch_merkle_layers(h->d_layers, N, s);
+  uint8_t* pin = rc == ZP_OK ? pinned_scratch(32) : nullptr;
+  if (rc == ZP_OK && !pin) rc = fail(ZP_ERR_OOM, "pinned staging allocation failed");
+  if (rc == ZP_OK && hipMemcpyAsync(pin, h->d_layers + (2 * N - 2) * 32, 32,
+                                    hipMemcpyDeviceToHost, s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "root copy failed");
+  if (rc == ZP_OK && hipStreamSynchronize(s) != hipSuccess)
+    rc = fail(ZP_ERR_DEVICE, "stream synchronize failed");
+  if (rc != ZP_OK) {  // copies may still be in flight on either stream
+    if (cs) (void)hipStreamSynchronize(cs);
+    (void)hipStreamSynchronize(s);
+  }
+  if (rc == ZP_OK) std::memcpy(h_root, pin, 32);
+  if (d_in) dev_free(d_in, s);
   if (rc != ZP_OK) {
     zp_hint_free(h);
     return rc;

@@ -639,15 +639,22 @@ class ZipPlus:
                                           stream=stream)
 
     def commit(self, pp, polys, stream=None):
-        """ZipPlus::commit (phase_commit.rs:61-101) -> (hint, commitment)."""
+        """ZipPlus::commit (phase_commit.rs:61-101) -> (hint, commitment).
+
+        Device tensors commit in place (zp_commit); host tensors, as the
+        reference's callers hold them, go through zp_commit_host, which
+        overlaps their upload with the encoding."""
         t = self._stack(pp, polys)
         validate_input(pp.linear_code.REPETITION_FACTOR, pp.linear_code.row_len(), self.types,
                        t.shape[0])
         h = ctypes.c_void_p()
         root = (ctypes.c_uint8 * 32)()
-        _check(L.load().zp_commit(pp.linear_code._h, self.types.code, self.types.degree_plus_one,
-                                  _dev_ptr(t), pp.num_rows, t.shape[0], ctypes.byref(h), root,
-                                  _stream(stream)))
+        if t.is_cuda:
+            fn, ptr = L.load().zp_commit, _dev_ptr(t)
+        else:
+            fn, ptr = L.load().zp_commit_host, ctypes.c_void_p(t.data_ptr())
+        _check(fn(pp.linear_code._h, self.types.code, self.types.degree_plus_one, ptr,
+                  pp.num_rows, t.shape[0], ctypes.byref(h), root, _stream(stream)))
         hint = ZipPlusHint(h, self.types, pp.linear_code, pp.num_rows)
         return hint, ZipPlusCommitment(bytes(root), t.shape[0])
 

@@ -85,6 +85,8 @@ SIGNATURES = {
     "zp_merkle_root": (_int, [_c_void_p, _u64, _u64, _u32, _c_void_p, _c_void_p]),
     "zp_commit": (_int, [_c_void_p, _int, _u32, _c_void_p, _u64, _u32,
                          ctypes.POINTER(_c_void_p), _c_void_p, _c_void_p]),
+    "zp_commit_host": (_int, [_c_void_p, _int, _u32, _c_void_p, _u64, _u32,
+                              ctypes.POINTER(_c_void_p), _c_void_p, _c_void_p]),
     "zp_commit_noalloc": (_int, [_c_void_p, _int, _u32, _c_void_p, _u64, _u32, _c_void_p,
                                  _c_void_p, _c_void_p]),
     "zp_hint_info": (_int, [_c_void_p, ctypes.POINTER(_u64), ctypes.POINTER(_u32),
