@@ -212,11 +212,12 @@ int zp_commit(const zp_code* code, int eval_type, uint32_t degree_plus_one, cons
 
 /* ZipPlus::commit(pp, polys) (phase_commit.rs:61-101) with the evaluations
  * in host memory, as the reference's callers hold them: the same commit as
- * zp_commit of the same bytes.  h_evals (pinned or pageable) is copied to
- * the device in up to four row groups per poly on a second stream, and each
- * group is encoded on `stream` as soon as its rows have landed, so the copy
- * of one group overlaps the encoding of the groups before it.  Returns when
- * the root is back; h_evals may be reused then. */
+ * zp_commit of the same bytes.  Pageable h_evals is copied to the device
+ * in four row groups per poly (polys of 4 rows or more) on a second
+ * stream, and each group is encoded on `stream` as soon as its rows have
+ * landed, so the copy of one group overlaps the encoding of the groups
+ * before it.  Pinned h_evals (hipHostMalloc) is copied once, on `stream`.
+ * Returns when the root is back; h_evals may be reused then. */
 int zp_commit_host(const zp_code* code, int eval_type, uint32_t degree_plus_one,
                    const void* h_evals, uint64_t num_rows, uint32_t batch, zp_hint** hint_out,
                    uint8_t h_root[32], zp_stream stream);
