@@ -575,7 +575,7 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   uint8_t* pin = pinned_scratch(std::max(up1 + down1, up2 + down2));
   if (!pin) return done(fail(ZP_ERR_OOM, "pinned staging allocation failed"));
 
-  // Round 1: q tables, plane sums, b.
+  // First prover message: q tables, plane sums, b.
   std::memcpy(pin, rr.data(), rr.size() * sizeof(Fp));
   std::memcpy(pin + rr.size() * sizeof(Fp), alm.data(), alm.size() * sizeof(Fp));
   if (hipMemcpyAsync(d_rr, pin, rr.size() * sizeof(Fp), hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -640,7 +640,7 @@ extern "C" int zp_prove(const zp_code* code, int eval_type, uint32_t D, uint32_t
   std::vector<uint64_t> cols(num_openings);
   for (uint32_t k = 0; k < num_openings; k++) cols[k] = ts_squeeze_idx(t, N);
 
-  // Round 2: combined row, columns, paths.
+  // Second prover message: combined row, columns, paths.
   uint8_t* up = pin;
   std::memcpy(up, al.data(), al.size() * 8);
   std::memcpy(up + al.size() * 8, co.data(), num_rows * 16);
